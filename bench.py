@@ -1,0 +1,255 @@
+"""Benchmark: probe·slice steps/s of the batched CN slice stepping, forward + adjoint.
+
+Workload (one "step" of this benchmark = one pass of the hot path over the
+whole batch): 2D CN stepping with per-slice tridiagonal LU on four scans in one
+persistent launch per direction -- Nx = 1024 with 512 probes plus Nx = 384,
+640, 896 with 128 probes each, Nz = 1024 (1023 slices), Dirichlet rows,
+complex64 data with FP64 pivots.  A pass factors every slice of every scan for
+the forward and for the adjoint operator and sweeps all probes forward and
+adjoint, writing the full complex64 field (916,608 forward + 916,608 adjoint
+probe·slice steps).  Inputs (object coefficients, probes) are resident in HBM
+before the timed region.
+
+Also reported beside the headline: the plain form (one launch per slice), the
+multislice propagator on the main scan (1024-point FFT per slice), the
+roofline of the dominant kernel (the persistent sweep) and the FP64 CPU
+oracle timed on a bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true", help="skip plain-form and multislice lines")
+    ap.add_argument("--cpu-probes", type=int, default=12)
+    ap.add_argument("--ppb", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: every box in this environment has a single MI355X and the probe-sharded "
+            "multi-GPU path is not built (see DESIGN.md for how it would shard).",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+    import torch
+
+    from thick_ptycho_amd import workloads
+    from thick_ptycho_amd.forward_model import ms_engine, operators, tri_engine
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    wl = workloads.config1(nz=args.nz)
+    log(f"workload {wl.name}: {wl.steps} forward steps per direction")
+
+    scans = []
+    for sc in wl.scans:
+        A, B = operators.step_operators_1d(sc.nx, sc.k, sc.dx, sc.dz, sc.bc)
+        k, dz = sc.k, sc.dz
+        coef = (k / 1j) * (sc.n - 1)
+        C = (dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2
+        scans.append(
+            dict(
+                sc=sc,
+                A=A,
+                B=B,
+                C=torch.from_numpy(np.ascontiguousarray(C)).to(dev),
+                u0=torch.from_numpy(sc.probes).to(dev, torch.complex64).contiguous(),
+                out_f=torch.empty((sc.probes.shape[0], sc.nz, sc.nx), dtype=torch.complex64, device=dev),
+                out_a=torch.empty((sc.probes.shape[0], sc.nz, sc.nx), dtype=torch.complex64, device=dev),
+            )
+        )
+    chunk = 16
+    steps_per_dir = sum(s["sc"].steps for s in scans)
+    steps_per_pass = 2 * steps_per_dir
+
+    def one_pass(plain: bool = False):
+        for mode, key in (("forward", "out_f"), ("adjoint", "out_a")):
+            specs = []
+            for s in scans:
+                fac = tri_engine.factor(s["C"], s["A"], s["B"], mode, chunk=chunk)
+                specs.append(tri_engine.TriScanSpec(fac, s["u0"], out=s[key], flip=(mode == "adjoint")))
+            if plain:
+                tri_engine.sweep_plain(specs)
+            else:
+                tri_engine.sweep(specs, probes_per_block=args.ppb)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    log(f"persistent batched form: warmup {args.warmup}, steps {args.steps}")
+    t_pass = timed(one_pass, args.steps, args.warmup)
+    value = steps_per_pass / t_pass
+    log(f"persistent: {t_pass * 1e3:.3f} ms/pass, {value:.4g} steps/s")
+
+    # ---- dominant kernel: the persistent forward sweep launch, timed with HIP events
+    facs = [tri_engine.factor(s["C"], s["A"], s["B"], "forward", chunk=chunk) for s in scans]
+    specs = [tri_engine.TriScanSpec(f, s["u0"], out=s["out_f"]) for f, s in zip(facs, scans)]
+    tri_engine.sweep(specs, probes_per_block=args.ppb)
+    torch.cuda.synchronize()
+    reps = max(3, args.steps)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        tri_engine.sweep(specs, probes_per_block=args.ppb)
+    e1.record()
+    torch.cuda.synchronize()
+    t_kernel = e0.elapsed_time(e1) / 1e3 / reps
+    alg_bytes = 0
+    for s in scans:
+        P, N, L = s["sc"].probes.shape[0], s["sc"].nx, s["sc"].nz - 1
+        alg_bytes += 8 * N * P * L + 24 * N * L + 8 * N * P
+    achieved = alg_bytes / t_kernel / 1e9
+    log(f"sweep kernel: {t_kernel * 1e3:.3f} ms/launch, {achieved:.1f} GB/s algorithmic")
+
+    # ---- factor kernel share
+    e0.record()
+    for _ in range(reps):
+        for s in scans:
+            tri_engine.factor(s["C"], s["A"], s["B"], "forward", chunk=chunk)
+    e1.record()
+    torch.cuda.synchronize()
+    t_factor = e0.elapsed_time(e1) / 1e3 / reps
+
+    extras = {}
+    if not args.no_extras:
+        log("plain form (one launch per slice)")
+        t_plain = timed(lambda: one_pass(plain=True), max(1, args.steps // 2), 1)
+        extras["plain_value"] = steps_per_pass / t_plain
+        extras["plain_ms_per_step"] = t_plain * 1e3
+        log(f"plain: {t_plain * 1e3:.3f} ms/pass, {extras['plain_value']:.4g} steps/s")
+
+        main = scans[0]
+        sc = main["sc"]
+        n_dev = torch.from_numpy(np.ascontiguousarray(sc.n)).to(dev)
+        H = ms_engine.propagator(sc.nx, sc.dx, sc.k, sc.dz, dev)
+        ms_out = torch.empty((sc.probes.shape[0], sc.nz, sc.nx), dtype=torch.complex64, device=dev)
+
+        def ms_pass(plain=False):
+            T = ms_engine.transmission(n_dev, sc.k, sc.dz, 1.0)
+            spec = [ms_engine.MsScanSpec(T, H, main["u0"], out=ms_out)]
+            (ms_engine.sweep_plain if plain else ms_engine.sweep)(spec)
+
+        t_ms = timed(ms_pass, args.steps, 1)
+        extras["multislice_value"] = sc.steps / t_ms
+        t_ms_plain = timed(lambda: ms_pass(True), max(1, args.steps // 2), 1)
+        extras["multislice_plain_value"] = sc.steps / t_ms_plain
+        log(f"multislice: {extras['multislice_value']:.4g} steps/s persistent, "
+            f"{extras['multislice_plain_value']:.4g} plain")
+        del ms_out
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl.scans[0], args.cpu_probes)
+        log(f"cpu baseline: {cpu['value']:.4g} steps/s ({cpu['sample']})")
+
+    line = {
+        "metric": "probe·slice steps/s fwd+adj, batched/learned vs plain, ≤1e-4 rel-L2 vs FP64",
+        "value": value,
+        "unit": "probe·slice steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_pass * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64 (FP64 pivots)",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D CN slice stepping, batched persistent form, forward+adjoint incl. per-slice factorisation",
+            "scans": [f"{s['sc'].nx}x{s['sc'].nz} x {s['sc'].probes.shape[0]} probes" for s in scans],
+            "steps_per_pass": steps_per_pass,
+            "bc": "dirichlet",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"tri_sweep_kernel<16,{args.ppb}> (forward, 4 scans)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "launch_ms": t_kernel * 1e3,
+            "algorithmic_bytes": alg_bytes,
+        },
+        "factor_ms_per_direction": t_factor * 1e3,
+        "sweep_only_value": 2 * steps_per_dir / (2 * t_kernel),
+        **extras,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(sc, probes: int) -> dict:
+    """FP64 oracle on the host: per-slice SuperLU factorisation and, per probe
+    and slice, a CSR mat-vec plus a SuperLU solve (the reference's loop),
+    forward and adjoint, on the first ``probes`` probes of the main scan."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import pwe
+
+    g = pwe.Grid(dimension=2, nx=sc.nx, nz=sc.nz, dx=sc.dx, dz=sc.dz, k=sc.k)
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        for mode in ("forward", "adjoint"):
+            cache = pwe.build_lu_cache(g, sc.bc, sc.n, mode)
+            for p in range(probes):
+                pwe.solve_single_probe(g, cache, sc.probes[p], mode)
+        dt = time.perf_counter() - t0
+    steps = 2 * probes * (sc.nz - 1)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": steps / dt,
+        "unit": "probe·slice steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{probes} probes x {sc.nz - 1} slices of the {sc.nx}-wide scan, forward+adjoint incl. "
+                  f"factorisation, complex128, 1 thread ({cpu_model}; host has {os.cpu_count()} cpus)",
+        "seconds": dt,
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,165 @@
+/*
+ * thick_ptycho_amd — C ABI of the MI355X (gfx950) kernels for thick-sample
+ * ptychography forward models: Crank–Nicolson slice stepping of the paraxial
+ * wave equation (per-slice tridiagonal LU), its adjoint, and the
+ * angular-spectrum multislice propagator.
+ *
+ * Conventions
+ *   - Every function returns int: TP_OK (0) or a negative TP_ERR_* code;
+ *     tp_last_error() returns a message for the calling thread.
+ *   - All pointers named d_* are device pointers owned by the caller; complex
+ *     values are interleaved (re, im): complex64 = 2 x float, complex128 =
+ *     2 x double.  The library never allocates device memory.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = default stream) and
+ *     is asynchronous; calls are capturable into HIP graphs.
+ *
+ * Reference interfaces replaced (paths relative to bryceshirley/thick_ptycho):
+ *   tp_tri_factor   <- PWEIterativeLUSolver._construct_solve_cache /
+ *                      _build_lu_factors (thick_ptycho/forward_model/pwe/solvers/
+ *                      iterative_lu.py:66-133): spla.splu of A0 - D_j per slice.
+ *   tp_tri_sweep    <- PWEIterativeLUSolver._solve_single_probe_impl
+ *                      (iterative_lu.py:138-197) driven by
+ *                      BaseForwardModelSolver.solve (forward_model/base/
+ *                      base_solver.py:84-154): u_j = LU_j^{-1}(B_j u_{j-1} + b).
+ *   tp_tri_step     <- one iteration j of the loop at iterative_lu.py:185-193
+ *                      for every probe (the "plain" one-launch-per-slice form).
+ *   tp_tri_gradient <- the adjoint sweep plus gradient accumulation of
+ *                      ReconstructorPWE.compute_grad_least_squares
+ *                      (thick_ptycho/reconstruction/pwe_reconstructor.py:100-164).
+ *   tp_ms_transmission, tp_ms_sweep
+ *                   <- MSForwardModelSolver._solve_single_probe
+ *                      (thick_ptycho/forward_model/multislice/ms_solver.py:90-159)
+ *                      with _propagate / _get_propagation_kernels (:49-78).
+ *   tp_farfield     <- BaseForwardModelSolver.get_farfield_intensities
+ *                      (forward_model/base/base_solver.py:213-249), noise off.
+ */
+#ifndef THICK_PTYCHO_AMD_H
+#define THICK_PTYCHO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TP_ABI_VERSION 1
+
+#define TP_OK 0
+#define TP_ERR_INVALID (-1)     /* bad argument or unsupported size      */
+#define TP_ERR_HIP (-2)         /* a HIP runtime call failed              */
+#define TP_ERR_SINGULAR (-3)    /* zero pivot in a per-slice factor       */
+#define TP_ERR_UNSUPPORTED (-4) /* size/mode not built into this library  */
+
+/* Maximum number of scans (objects of possibly different width) one
+ * tp_tri_sweep / tp_ms_sweep launch walks together. */
+#define TP_MAX_SCANS 8
+
+int tp_abi_version(void);
+const char* tp_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Tridiagonal CN stepping (one transverse axis)
+ *
+ * Rows of the per-slice systems are split into 64*chunk lanes; `chunk` is the
+ * number of consecutive rows one GPU lane owns (1, 2, 4, 8, 16 or 32) and must
+ * satisfy n <= 64*chunk.  n_pad = 64*chunk.
+ *
+ * Coefficient buffer (`d_coef`): complex64 [steps][6][n_pad], permuted so that
+ * row i is stored at (i % chunk) * 64 + i / chunk.  Its size in bytes is
+ * steps * 6 * n_pad * 8 (tp_tri_coef_bytes).
+ * ------------------------------------------------------------------------ */
+
+size_t tp_tri_coef_bytes(int n, int steps, int chunk);
+
+/* Factor, for every step s = 1..steps, the LHS tridiagonal
+ *     lhs(s) = tridiag(lhs_sub, lhs_diag + lhs_sign * c_l(s), lhs_sup)
+ * and record the RHS diagonal rhs_diag + rhs_sign * c_r(s), where
+ *     c_l(s) = C[:, lhs_col0 + col_step*(s-1)],  c_r(s) = C[:, rhs_col0 + col_step*(s-1)]
+ * (a column outside [0, ncol) contributes 0; conj_c conjugates C first).
+ * d_C: complex128, row-major (n, ncol) = (n, nz-1) exactly as returned by
+ *      create_object_contribution(...).reshape(-1, nz-1).
+ * d_base: complex128 [6][n] = lhs_sub, lhs_diag, lhs_sup, rhs_sub, rhs_diag, rhs_sup
+ *      (sub[0] and sup[n-1] are ignored).
+ * The pivots are computed in FP64 (no pivot search: the CN operators are
+ * strictly diagonally dominant); products are stored in complex64.
+ * Returns TP_ERR_SINGULAR if a pivot is exactly zero (checked on the host
+ * only when `d_status` is read back by the caller: d_status[0] != 0). */
+int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_base,
+                  double lhs_sign, double rhs_sign, int conj_c,
+                  int lhs_col0, int rhs_col0, int col_step, int steps,
+                  int chunk, void* d_coef, int* d_status, void* stream);
+
+/* RHS off-diagonals in the permuted layout: complex64 [2][n_pad]
+ * (sub, sup), rows >= n are zero.  Built by the caller (host) or with
+ * tp_tri_offdiag from complex128 arrays. */
+int tp_tri_offdiag(const void* d_base, int n, int chunk, void* d_offd, void* stream);
+
+/* One scan of a multi-scan sweep. All strides are in complex elements. */
+typedef struct tp_tri_scan {
+  const void* coef;       /* d_coef of this scan (step 1 first)                 */
+  const void* offd;       /* d_offd of this scan                                */
+  const void* u0;         /* initial planes: complex64, u0 + p*u0_pstride       */
+  const void* rhs;        /* optional extra RHS per step, or NULL:
+                             rhs + p*rhs_pstride + (s-1)*rhs_sstride, complex64 */
+  void* out;              /* full field or NULL: plane s at out + p*out_pstride +
+                             (out_z0 + out_zdir*s)*out_zstride, s = 0..steps     */
+  void* ckpt;             /* checkpoints or NULL: plane s/ckpt_every for
+                             s % ckpt_every == 0 at ckpt + p*ckpt_pstride +
+                             (s/ckpt_every)*ckpt_zstride                         */
+  void* exitp;            /* last plane or NULL: exitp + p*exit_pstride          */
+  int64_t u0_pstride, rhs_pstride, rhs_sstride;
+  int64_t out_pstride, out_zstride, ckpt_pstride, ckpt_zstride, exit_pstride;
+  int n, steps, probes;
+  int out_z0, out_zdir, ckpt_every;
+} tp_tri_scan;
+
+/* Persistent batched sweep: one launch walks all `steps` slices of every probe
+ * of every scan (nscans <= TP_MAX_SCANS, all with n <= 64*chunk). The
+ * per-slice coefficients are staged in LDS and shared by the
+ * `probes_per_block` probes (1, 2, 4 or 8) of one workgroup. */
+int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int probes_per_block, void* stream);
+
+/* Plain form: advance every probe of every scan by the single step `s`
+ * (1-based), reading plane s-1 from `out` (or u0 when s == 1, which also
+ * writes plane 0) and writing plane s.  Requires `out`. */
+int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int s, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Multislice (angular spectrum), n = 64 * r with r in {4, 8, 16, 32}
+ * ------------------------------------------------------------------------ */
+
+/* T[z][x] = exp(i k (n[x, z] - n_medium) dz) for z = 0..nz-2, complex64.
+ * d_n: complex128 row-major (n, nz). */
+int tp_ms_transmission(const void* d_n, int n, int nz, double k, double dz, double n_medium,
+                       void* d_T, void* stream);
+
+/* Propagator kernel H(kx) * (1/n) in natural frequency order, complex64 [n]. */
+int tp_ms_kernel(int n, double dx, double k, double dz, void* d_H, void* stream);
+
+typedef struct tp_ms_scan {
+  const void* T;          /* complex64 [steps][n] transmission per slice          */
+  const void* H;          /* complex64 [n] = H / n                                */
+  const void* u0;         /* complex64 initial planes, u0 + p*u0_pstride          */
+  void* out;              /* full field or NULL (plane z at out + p*out_pstride + z*out_zstride) */
+  void* exitp;            /* last plane or NULL                                   */
+  int64_t u0_pstride, out_pstride, out_zstride, exit_pstride;
+  int n, steps, probes, conj_h;  /* conj_h: back-propagation with conj(H)     */
+} tp_ms_scan;
+
+/* Persistent multislice: psi <- ifft(fft(psi * T_z) * H) for z = 0..steps-1,
+ * one wavefront per probe. */
+int tp_ms_sweep(const tp_ms_scan* scans, int nscans, void* stream);
+
+/* Plain form: one slice z (0-based) for every probe, reading plane z from
+ * `out` (u0 when z == 0, also writing plane 0) and writing plane z+1. */
+int tp_ms_step(const tp_ms_scan* scans, int nscans, int z, void* stream);
+
+/* Far-field intensities |fft(exit)|^2 (unnormalised, no shift), float32
+ * [probes][n], for exit waves complex64 [probes][n] (row stride ld). */
+int tp_farfield(const void* d_exit, int64_t ld, int n, int probes, void* d_intensity, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THICK_PTYCHO_AMD_H */

@@ -1,0 +1,108 @@
+"""FP64 restatement of the reference's multislice propagator and far field.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+* ``thick_ptycho/forward_model/multislice/ms_solver.py:49-159``
+* ``thick_ptycho/forward_model/base/base_solver.py:194-249``
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+
+def propagation_kernel(nx: int, dx: float, k: float, dz: float, remove_global_phase=True) -> np.ndarray:
+    """``_get_propagation_kernels`` (ms_solver.py:49-72).
+
+    The reference passes ``psi.size`` as ``remove_global_phase`` (always truthy,
+    quirk Q6), so the global phase exp(-i k dz) is always removed and the
+    evanescent band is clipped to kz = 0."""
+    fx = np.fft.fftfreq(nx, d=dx)
+    kx = 2 * np.pi * fx
+    inside = k**2 - kx**2
+    kz = np.sqrt(np.clip(inside, 0.0, None))
+    H = np.exp(1j * kz * dz)
+    if remove_global_phase:
+        H *= np.exp(-1j * k * dz)
+    return H.astype(np.complex128)
+
+
+def transmission(n_slice: np.ndarray, k: float, dz: float, n_medium: float) -> np.ndarray:
+    """``_object_transmission_function`` (ms_solver.py:86-88)."""
+    return np.exp(1j * k * (n_slice - n_medium) * dz)
+
+
+def solve_single_probe(n: np.ndarray, probe: np.ndarray, dx: float, k: float, dz: float, n_medium: float = 1.0) -> np.ndarray:
+    """``MSForwardModelSolver._solve_single_probe`` (ms_solver.py:90-159): (nx, nz)."""
+    nx, nz = n.shape
+    H = propagation_kernel(nx, dx, k, dz, True)
+    psi = np.asarray(probe, dtype=complex).copy()
+    u = np.empty((nx, nz), dtype=complex)
+    u[:, 0] = psi
+    for z in range(nz - 1):
+        psi = psi * transmission(n[:, z], k, dz, n_medium)
+        psi = np.fft.ifft(np.fft.fft(psi) * H)
+        u[:, z + 1] = psi
+    return u
+
+
+def solve_batched(n: np.ndarray, probes: np.ndarray, dx: float, k: float, dz: float, n_medium: float = 1.0, keep: str = "all") -> np.ndarray:
+    """The loop above with the probe index vectorised: (P, nx, nz) or exit (P, nx)."""
+    nx, nz = n.shape
+    H = propagation_kernel(nx, dx, k, dz, True)
+    psi = np.asarray(probes, dtype=complex).copy()
+    P = psi.shape[0]
+    out = None
+    if keep == "all":
+        out = np.empty((P, nx, nz), dtype=complex)
+        out[:, :, 0] = psi
+    for z in range(nz - 1):
+        psi = psi * transmission(n[:, z], k, dz, n_medium)[None, :]
+        psi = np.fft.ifft(np.fft.fft(psi, axis=-1) * H[None, :], axis=-1)
+        if keep == "all":
+            out[:, :, z + 1] = psi
+    return psi if keep == "exit" else out
+
+
+def backpropagate(psi: np.ndarray, dx: float, k: float, dz: float) -> np.ndarray:
+    """``_backpropagate`` (ms_solver.py:80-84): ifft(fft(psi) * conj(H))."""
+    H = propagation_kernel(psi.shape[-1], dx, k, dz, True)
+    return np.fft.ifft(np.fft.fft(psi, axis=-1) * np.conj(H), axis=-1)
+
+
+# ----------------------------------------------------------------------------
+# Exit waves and far field  (base/base_solver.py:194-249)
+# ----------------------------------------------------------------------------
+
+
+def exit_waves(u: np.ndarray, total_scans: int, block_size: int) -> np.ndarray:
+    """``get_exit_waves``: u[..., -1] reshaped to (total_scans, block_size)."""
+    return u[..., -1].reshape((total_scans, block_size))
+
+
+def farfield_intensities(exit_w: np.ndarray, dimension: int, total_scans: int, block_size: int,
+                         rng: Optional[np.random.Generator] = None, poisson_noise: bool = False) -> np.ndarray:
+    """``get_farfield_intensities``: |fft(exit)|^2, unnormalised, no shift.
+
+    For ``dimension == 3`` the reference applies ``fft2`` over the last two
+    axes of the flattened (scans, N) array, i.e. across the scan axis (quirk
+    Q9); this restates that literally.  ``farfield_per_probe_2d`` is the
+    per-probe variant the HIP library computes for two transverse axes."""
+    if dimension == 2:
+        f = np.fft.fft(exit_w)
+    else:
+        f = np.fft.fft2(exit_w, axes=(-2, -1))
+    inten = np.abs(f) ** 2
+    if poisson_noise:
+        rng = rng or np.random.default_rng()
+        inten = rng.poisson(inten)
+    return inten.reshape((total_scans, block_size))
+
+
+def farfield_per_probe_2d(exit_w: np.ndarray, nx: int, ny: int) -> np.ndarray:
+    """|fft2|^2 of each probe's (nx, ny) exit wave, flattened back to (P, nx*ny)."""
+    P = exit_w.shape[0]
+    f = np.fft.fft2(exit_w.reshape(P, nx, ny), axes=(-2, -1))
+    return (np.abs(f) ** 2).reshape(P, nx * ny)

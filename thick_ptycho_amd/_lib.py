@@ -1,0 +1,145 @@
+"""ctypes binding of ``libtp_hip.so`` (C ABI in ``include/thick_ptycho_amd.h``).
+
+The library is built in-tree by ``python -m thick_ptycho_amd.build``.  There is
+no CPU fallback: if the library or a GPU is missing every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtp_hip.so")
+
+TP_OK = 0
+TP_ERR_INVALID = -1
+TP_ERR_HIP = -2
+TP_ERR_SINGULAR = -3
+TP_ERR_UNSUPPORTED = -4
+TP_MAX_SCANS = 8
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
+
+
+class TriScan(ctypes.Structure):
+    _fields_ = [
+        ("coef", c_void_p),
+        ("offd", c_void_p),
+        ("u0", c_void_p),
+        ("rhs", c_void_p),
+        ("out", c_void_p),
+        ("ckpt", c_void_p),
+        ("exitp", c_void_p),
+        ("u0_pstride", c_int64),
+        ("rhs_pstride", c_int64),
+        ("rhs_sstride", c_int64),
+        ("out_pstride", c_int64),
+        ("out_zstride", c_int64),
+        ("ckpt_pstride", c_int64),
+        ("ckpt_zstride", c_int64),
+        ("exit_pstride", c_int64),
+        ("n", c_int),
+        ("steps", c_int),
+        ("probes", c_int),
+        ("out_z0", c_int),
+        ("out_zdir", c_int),
+        ("ckpt_every", c_int),
+    ]
+
+
+class MsScan(ctypes.Structure):
+    _fields_ = [
+        ("T", c_void_p),
+        ("H", c_void_p),
+        ("u0", c_void_p),
+        ("out", c_void_p),
+        ("exitp", c_void_p),
+        ("u0_pstride", c_int64),
+        ("out_pstride", c_int64),
+        ("out_zstride", c_int64),
+        ("exit_pstride", c_int64),
+        ("n", c_int),
+        ("steps", c_int),
+        ("probes", c_int),
+        ("conj_h", c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tp_abi_version": (c_int, []),
+    "tp_last_error": (ctypes.c_char_p, []),
+    "tp_tri_coef_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "tp_tri_factor": (
+        c_int,
+        [c_void_p, c_int, c_int, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int,
+         c_void_p, c_void_p, c_void_p],
+    ),
+    "tp_tri_offdiag": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "tp_tri_sweep": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
+    "tp_tri_step": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
+    "tp_ms_transmission": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p, c_void_p]),
+    "tp_ms_kernel": (c_int, [c_int, c_double, c_double, c_double, c_void_p, c_void_p]),
+    "tp_ms_sweep": (c_int, [ctypes.POINTER(MsScan), c_int, c_void_p]),
+    "tp_ms_step": (c_int, [ctypes.POINTER(MsScan), c_int, c_int, c_void_p]),
+    "tp_farfield": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            f"{LIB_PATH} not found: build it with `python -m thick_ptycho_amd.build` (no CPU fallback exists)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == TP_OK:
+        return
+    msg = f"{what}: {load().tp_last_error().decode(errors='replace')}"
+    if rc in (TP_ERR_INVALID, TP_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def require_gpu() -> None:
+    import torch
+
+    load()
+    if not torch.cuda.is_available():
+        raise RuntimeError("thick_ptycho_amd needs a ROCm GPU (torch.cuda.is_available() is False); no CPU fallback")

@@ -1,0 +1,163 @@
+"""Device-side plumbing for the 2D (one transverse axis) CN stepping kernels.
+
+``factor`` runs ``tp_tri_factor`` for one (object, mode) and keeps the
+per-step coefficients on the GPU; ``TriScanSpec`` describes one batch of
+probes stepped through those factors; ``sweep`` launches the persistent
+batched kernel over up to eight such batches (scans of different widths in
+one launch) and ``step`` the plain one-launch-per-slice form.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .operators import ModePlan, Tridiag, chunk_for, mode_plan
+
+
+@dataclass
+class TriFactors:
+    coef: torch.Tensor
+    offd: torch.Tensor
+    n: int
+    steps: int
+    chunk: int
+    mode: str
+    b_sign: float
+    status: torch.Tensor
+    keep: list = field(default_factory=list)
+
+    def check(self) -> None:
+        """Raise if a pivot was exactly zero (synchronises)."""
+        if int(self.status.item()) != 0:
+            raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
+
+
+def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[int] = None,
+           stream=None) -> TriFactors:
+    """Factor every slice's LHS for ``mode``.
+
+    C: complex128 CUDA tensor (n, L) from create_object_contribution(...).reshape(-1, nz-1)."""
+    assert C.is_cuda and C.dtype == torch.complex128 and C.dim() == 2
+    n, L = C.shape
+    C = C.contiguous()
+    chunk = chunk or chunk_for(n)
+    plan: ModePlan = mode_plan(A, B, mode, L)
+    dev = C.device
+    base = torch.from_numpy(plan.base_array().astype(np.complex128)).to(dev)
+    coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=dev)
+    offd = torch.empty(2 * 64 * chunk, dtype=torch.complex64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    sh = _lib.stream_handle(stream)
+    _lib.check(
+        lib.tp_tri_factor(_lib.ptr(C), n, L, _lib.ptr(base), plan.lhs_sign, plan.rhs_sign, plan.conj_c,
+                          plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, _lib.ptr(coef),
+                          _lib.ptr(status), sh),
+        "tp_tri_factor",
+    )
+    _lib.check(lib.tp_tri_offdiag(_lib.ptr(base), n, chunk, _lib.ptr(offd), sh), "tp_tri_offdiag")
+    return TriFactors(coef, offd, n, L, chunk, mode, plan.b_sign, status, keep=[base, C])
+
+
+@dataclass
+class TriScanSpec:
+    """One batch of probes stepped through ``fac``.
+
+    u0:   complex64 (P, n) initial planes (contiguous rows).
+    out:  complex64 (P, L+1, n) full field, or None.
+    exit: complex64 (P, n) last plane, or None.
+    ckpt: complex64 (P, L//every + 1, n) checkpoints, or None.
+    rhs:  complex64 (P or 1, L, n) extra right-hand side per step (step order), or None.
+    flip: write plane s at index L - s (the reference adjoint's output flip).
+    """
+
+    fac: TriFactors
+    u0: torch.Tensor
+    out: Optional[torch.Tensor] = None
+    exit: Optional[torch.Tensor] = None
+    ckpt: Optional[torch.Tensor] = None
+    ckpt_every: int = 0
+    rhs: Optional[torch.Tensor] = None
+    flip: bool = False
+
+    def struct(self) -> _lib.TriScan:
+        f = self.fac
+        P = self.u0.shape[0]
+        n, L = f.n, f.steps
+        s = _lib.TriScan()
+        s.coef = _lib.ptr(f.coef)
+        s.offd = _lib.ptr(f.offd)
+        assert self.u0.dtype == torch.complex64 and self.u0.shape[-1] == n and self.u0.stride(-1) == 1
+        s.u0 = _lib.ptr(self.u0)
+        s.u0_pstride = self.u0.stride(0) if P > 1 else n
+        if self.rhs is not None:
+            assert self.rhs.dtype == torch.complex64 and self.rhs.shape[-2:] == (L, n) and self.rhs.stride(-1) == 1
+            s.rhs = _lib.ptr(self.rhs)
+            s.rhs_pstride = self.rhs.stride(0) if self.rhs.shape[0] > 1 else 0
+            s.rhs_sstride = self.rhs.stride(1)
+        if self.out is not None:
+            assert self.out.dtype == torch.complex64 and self.out.shape == (P, L + 1, n) and self.out.stride(-1) == 1
+            s.out = _lib.ptr(self.out)
+            s.out_pstride = self.out.stride(0)
+            s.out_zstride = self.out.stride(1)
+            s.out_z0 = L if self.flip else 0
+            s.out_zdir = -1 if self.flip else 1
+        if self.ckpt is not None:
+            assert self.ckpt_every >= 1 and self.ckpt.shape == (P, L // self.ckpt_every + 1, n)
+            s.ckpt = _lib.ptr(self.ckpt)
+            s.ckpt_pstride = self.ckpt.stride(0)
+            s.ckpt_zstride = self.ckpt.stride(1)
+            s.ckpt_every = self.ckpt_every
+        if self.exit is not None:
+            assert self.exit.dtype == torch.complex64 and self.exit.shape == (P, n) and self.exit.stride(-1) == 1
+            s.exitp = _lib.ptr(self.exit)
+            s.exit_pstride = self.exit.stride(0)
+        s.n = n
+        s.steps = L
+        s.probes = P
+        return s
+
+
+def _structs(specs: Sequence[TriScanSpec]):
+    if not 1 <= len(specs) <= _lib.TP_MAX_SCANS:
+        raise ValueError(f"between 1 and {_lib.TP_MAX_SCANS} scans per launch")
+    arr = (_lib.TriScan * len(specs))(*[s.struct() for s in specs])
+    chunk = max(s.fac.chunk for s in specs)
+    for s in specs:
+        if s.fac.chunk != chunk:
+            raise ValueError("all scans of one launch must be factored with the same chunk")
+    return arr, chunk
+
+
+def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) -> None:
+    """Persistent batched sweep over all steps of all scans (one launch)."""
+    arr, chunk = _structs(specs)
+    lib = _lib.load()
+    _lib.check(lib.tp_tri_sweep(arr, len(specs), chunk, probes_per_block, _lib.stream_handle(stream)), "tp_tri_sweep")
+
+
+def step(specs: Sequence[TriScanSpec], s: int, stream=None) -> None:
+    """Plain form: one launch advancing every probe by step ``s``."""
+    arr, chunk = _structs(specs)
+    lib = _lib.load()
+    _lib.check(lib.tp_tri_step(arr, len(specs), chunk, s, _lib.stream_handle(stream)), "tp_tri_step")
+
+
+def sweep_plain(specs: Sequence[TriScanSpec], stream=None) -> None:
+    """Plain form over all steps: one launch per slice."""
+    L = specs[0].fac.steps
+    for spec in specs:
+        if spec.fac.steps != L or spec.out is None:
+            raise ValueError("plain sweeps need equal step counts and full-field outputs")
+    arr, chunk = _structs(specs)
+    lib = _lib.load()
+    sh = _lib.stream_handle(stream)
+    n = len(specs)
+    for s in range(1, L + 1):
+        _lib.check(lib.tp_tri_step(arr, n, chunk, s, sh), "tp_tri_step")
