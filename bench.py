@@ -43,7 +43,7 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip plain-form and multislice lines")
-    ap.add_argument("--cpu-probes", type=int, default=12)
+    ap.add_argument("--cpu-probes", type=int, default=24)
     ap.add_argument("--ppb", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -1,0 +1,71 @@
+"""Pin the FP64 multislice / far-field restatement with closed forms (CPU)."""
+
+import numpy as np
+
+from oracle import multislice as oms
+from tests.cases import rel_l2
+
+
+def test_plane_waves_are_eigenvectors():
+    """exp(2 pi i m x / N) propagates by H(k_m) per slice in free space."""
+    N, nz, dx, dz, k = 128, 6, 0.4, 1.5, 2 * np.pi
+    x = np.arange(N)
+    for m in (0, 5, 40, -17):
+        v = np.exp(2j * np.pi * m * x / N)
+        u = oms.solve_single_probe(np.ones((N, nz)), v, dx, k, dz)
+        kx = 2 * np.pi * m / (N * dx)
+        kz = np.sqrt(max(k * k - kx * kx, 0.0))
+        lam = np.exp(1j * (kz - k) * dz)
+        for z in range(nz):
+            assert np.allclose(u[:, z], v * lam**z, atol=1e-12)
+
+
+def test_evanescent_band_clipped_not_decaying():
+    """kz is clipped to 0 beyond k (quirk Q6): |H| = 1 everywhere."""
+    H = oms.propagation_kernel(256, 0.1, 2 * np.pi, 1.0)
+    assert np.allclose(np.abs(H), 1.0)
+    kx = 2 * np.pi * np.fft.fftfreq(256, 0.1)
+    ev = np.abs(kx) > 2 * np.pi
+    assert ev.any() and np.allclose(H[ev], np.exp(-1j * 2 * np.pi * 1.0))
+
+
+def test_constant_object_phase_and_parseval():
+    N, nz, dx, dz, k = 64, 5, 1.0, 2.0, 2 * np.pi
+    rng = np.random.default_rng(0)
+    probe = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    dn = 3e-3 + 1e-4j
+    n = np.full((N, nz), 1 + dn)
+    u = oms.solve_single_probe(n, probe, dx, k, dz)
+    u0 = oms.solve_single_probe(np.ones((N, nz)), probe, dx, k, dz)
+    for z in range(nz):
+        assert np.allclose(u[:, z], u0[:, z] * np.exp(1j * k * dn * dz) ** z, atol=1e-12)
+        # free-space propagation is unitary
+        assert np.isclose(np.linalg.norm(u0[:, z]), np.linalg.norm(probe))
+
+
+def test_batched_equals_single_and_backprop_inverts():
+    N, nz, dx, dz, k = 96, 7, 0.7, 1.0, 2 * np.pi
+    rng = np.random.default_rng(1)
+    n = 1 - 1e-3 * rng.random((N, nz))
+    P = rng.standard_normal((3, N)) + 0j
+    b = oms.solve_batched(n, P, dx, k, dz)
+    for p in range(3):
+        assert rel_l2(b[p], oms.solve_single_probe(n, P[p], dx, k, dz)) < 1e-14
+    back = oms.backpropagate(oms.backpropagate(b[0][:, 0], dx, k, -dz), dx, k, dz)
+    assert rel_l2(back, b[0][:, 0]) < 1e-12
+
+
+def test_farfield_parseval_and_delta():
+    rng = np.random.default_rng(2)
+    ex = rng.standard_normal((4, 50)) + 1j * rng.standard_normal((4, 50))
+    I = oms.farfield_intensities(ex, 2, 4, 50)
+    assert np.allclose(I.sum(axis=1), 50 * np.sum(np.abs(ex) ** 2, axis=1))
+    d = np.zeros((1, 50), complex)
+    d[0, 0] = 2.0
+    assert np.allclose(oms.farfield_intensities(d, 2, 1, 50), 4.0)
+    # the literal two-axis restatement transforms across the scan axis (quirk Q9)
+    ex3 = rng.standard_normal((3, 16)) + 0j
+    q9 = oms.farfield_intensities(ex3, 3, 3, 16)
+    assert np.allclose(q9, np.abs(np.fft.fft2(ex3)) ** 2)
+    pp = oms.farfield_per_probe_2d(ex3, 4, 4)
+    assert np.allclose(pp[1], (np.abs(np.fft.fft2(ex3[1].reshape(4, 4))) ** 2).ravel())
