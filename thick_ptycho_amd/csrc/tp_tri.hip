@@ -106,51 +106,81 @@ __global__ void tri_offdiag_kernel(const double2* __restrict__ base, int n, int 
 }
 
 // ------------------------------------------------------------------ wave scans
-// Inclusive scan over lanes of affine maps f_l(y) = A_l y + B_l, composed
-// towards higher lanes (UP) or towards lower lanes (DOWN).  After the scan
-// lane l holds the composition f_l o f_{l-1} o ... o f_0 (UP).
-__device__ __forceinline__ float2 shfl_up2(float2 v, int d) {
-  return make_float2(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64));
+// Inclusive scans over the 64 lanes of affine maps f_l(y) = A_l y + B_l
+// (complex A, B), built from DPP row shifts.  UP composes towards higher lanes
+// (lane l ends with f_l o ... o f_0), DOWN towards lower lanes (lane l ends
+// with f_l o ... o f_63).  Lanes whose DPP source is outside the row keep the
+// identity map passed as the `old` operand.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f(float old, float src) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, ROW_MASK, 0xF, false));
 }
-__device__ __forceinline__ float2 shfl_down2(float2 v, int d) {
-  return make_float2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float2 dpp_c(float2 old, float2 src) {
+  return make_float2(dpp_f<CTRL, ROW_MASK>(old.x, src.x), dpp_f<CTRL, ROW_MASK>(old.y, src.y));
 }
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ void affine_step(float2& A, float2& B) {
+  const float2 Ap = dpp_c<CTRL, ROW_MASK>(make_float2(1.f, 0.f), A);
+  const float2 Bp = dpp_c<CTRL, ROW_MASK>(make_float2(0.f, 0.f), B);
+  B = cfma(A, Bp, B);
+  A = cmul(A, Ap);
+}
+// DPP controls (GFX9 encoding)
+constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL2 = 0x102, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHL8 = 0x108;
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
 __device__ __forceinline__ void scan_affine_up(float2& A, float2& B, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const float2 Ap = shfl_up2(A, d);
-    const float2 Bp = shfl_up2(B, d);
-    if (lane >= d) {
-      B = cfma(A, Bp, B);
-      A = cmul(A, Ap);
-    }
-  }
+  (void)lane;
+  affine_step<DPP_ROW_SHR1>(A, B);
+  affine_step<DPP_ROW_SHR2>(A, B);
+  affine_step<DPP_ROW_SHR4>(A, B);
+  affine_step<DPP_ROW_SHR8>(A, B);
+  affine_step<DPP_ROW_BCAST15, 0xA>(A, B);
+  affine_step<DPP_ROW_BCAST31, 0xC>(A, B);
 }
+
+__device__ __forceinline__ float2 readlane_c(float2 v, int l) {
+  return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l)));
+}
+
 __device__ __forceinline__ void scan_affine_down(float2& A, float2& B, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const float2 Ap = shfl_down2(A, d);
-    const float2 Bp = shfl_down2(B, d);
-    if (lane + d < 64) {
-      B = cfma(A, Bp, B);
-      A = cmul(A, Ap);
-    }
-  }
+  affine_step<DPP_ROW_SHL1>(A, B);
+  affine_step<DPP_ROW_SHL2>(A, B);
+  affine_step<DPP_ROW_SHL4>(A, B);
+  affine_step<DPP_ROW_SHL8>(A, B);
+  // lanes 16, 32, 48 now hold the totals of rows 1, 2, 3
+  const float2 A1 = readlane_c(A, 16), B1 = readlane_c(B, 16);
+  const float2 A2 = readlane_c(A, 32), B2 = readlane_c(B, 32);
+  const float2 A3 = readlane_c(A, 48), B3 = readlane_c(B, 48);
+  const float2 A23 = cmul(A2, A3), B23 = cfma(A2, B3, B2);
+  const float2 A123 = cmul(A1, A23), B123 = cfma(A1, B23, B1);
+  const int row = lane >> 4;
+  float2 TA = make_float2(1.f, 0.f), TB = make_float2(0.f, 0.f);
+  if (row == 0) { TA = A123; TB = B123; }
+  if (row == 1) { TA = A23; TB = B23; }
+  if (row == 2) { TA = A3; TB = B3; }
+  B = cfma(A, TB, B);
+  A = cmul(A, TA);
 }
+
+// previous lane's value (lane 0 gets 0) / next lane's value (lane 63 gets 0)
+__device__ __forceinline__ float2 from_prev_lane(float2 v) { return dpp_c<DPP_WAVE_SHR1>(make_float2(0.f, 0.f), v); }
+__device__ __forceinline__ float2 from_next_lane(float2 v) { return dpp_c<DPP_WAVE_SHL1>(make_float2(0.f, 0.f), v); }
 
 // One CN step for the CH rows of this lane.  `cf` points at the step's six
 // permuted coefficient arrays, `od` at the two permuted off-diagonal arrays
 // (LDS or global).  `rhs` (natural order rows of this lane) may be null.
-template <int CH, typename Ptr>
-__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, const float2* rhs, int row0, int n,
-                                              int lane) {
+template <int CH, bool HAS_RHS, typename Ptr>
+__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, const float2 (&rhs)[CH], int row0,
+                                              int n, int lane) {
   constexpr int NP = 64 * CH;
-  const float2 z2 = make_float2(0.f, 0.f);
-  float2 left = shfl_up2(u[CH - 1], 1);
-  float2 right = shfl_down2(u[0], 1);
-  if (lane == 0) left = z2;
-  if (lane == 63) right = z2;
+  const float2 left = from_prev_lane(u[CH - 1]);
+  const float2 right = from_next_lane(u[0]);
   float2 y[CH];
 #pragma unroll
   for (int m = 0; m < CH; ++m) {
@@ -160,10 +190,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, c
     float2 r = cmul(cf[A_BD * NP + k], u[m]);
     r = cfma(od[k], um, r);
     r = cfma(od[NP + k], up, r);
-    if (rhs != nullptr) {
-      const int i = row0 + m;
-      if (i < n) r = cadd(r, rhs[m]);
-    }
+    if (HAS_RHS) r = cadd(r, rhs[m]);
     y[m] = r;
   }
 #pragma unroll
@@ -172,8 +199,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, c
     float2 A = cf[A_GF * NP + (CH - 1) * 64 + lane];
     float2 B = y[CH - 1];
     scan_affine_up(A, B, lane);
-    float2 yin = shfl_up2(B, 1);
-    if (lane == 0) yin = z2;
+    const float2 yin = from_prev_lane(B);
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
   }
@@ -185,8 +211,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, c
     float2 A = cf[A_GB * NP + lane];
     float2 B = y[0];
     scan_affine_down(A, B, lane);
-    float2 xin = shfl_down2(B, 1);
-    if (lane == 63) xin = z2;
+    const float2 xin = from_next_lane(B);
 #pragma unroll
     for (int m = 0; m < CH; ++m) u[m] = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
   }
@@ -207,38 +232,43 @@ __device__ __forceinline__ int find_scan(const SweepArgs& a, int blk) {
   return sc;
 }
 
+// ------------------------------------------------------------------ persistent sweep
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <int CH>
-__device__ __forceinline__ void store_plane(const tp_tri_scan& d, int p, int s, const float2 (&u)[CH], int row0) {
-  const int n = d.n;
-  if (d.out) {
-    float2* o = (float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride;
+__device__ __forceinline__ void store_rows(float2* o, const float2 (&u)[CH], int row0, int n) {
+  if (row0 + CH <= n) {
 #pragma unroll
-    for (int m = 0; m < CH; ++m)
-      if (row0 + m < n) o[row0 + m] = u[m];
-  }
-  if (d.ckpt && (s % d.ckpt_every) == 0) {
-    float2* o = (float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride;
-#pragma unroll
-    for (int m = 0; m < CH; ++m)
-      if (row0 + m < n) o[row0 + m] = u[m];
-  }
-  if (d.exitp && s == d.steps) {
-    float2* o = (float2*)d.exitp + (size_t)p * d.exit_pstride;
+    for (int m = 0; m < CH; ++m) o[row0 + m] = u[m];
+  } else {
 #pragma unroll
     for (int m = 0; m < CH; ++m)
       if (row0 + m < n) o[row0 + m] = u[m];
   }
 }
 
-// ------------------------------------------------------------------ persistent sweep
+template <int CH>
+__device__ __forceinline__ void store_plane_fast(const tp_tri_scan& d, int p, int s, const float2 (&u)[CH], int row0) {
+  const int n = d.n;
+  if (d.out) store_rows<CH>((float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride, u, row0, n);
+  if (d.ckpt && (s % d.ckpt_every) == 0)
+    store_rows<CH>((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, u, row0, n);
+  if (d.exitp && s == d.steps) store_rows<CH>((float2*)d.exitp + (size_t)p * d.exit_pstride, u, row0, n);
+}
+
+// One workgroup = PPB wavefronts = PPB probes of one scan walking all steps in
+// lockstep.  The step's coefficients (6 * 64*CH complex64) are staged in LDS
+// and shared by the PPB wavefronts; the next step's block is fetched into
+// registers while the current step computes.  The plane computed in step s is
+// written at the start of step s+1 so that the stores drain behind the math.
 template <int CH, int PPB>
 __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
-  constexpr int NV = A_COUNT * NP / 2;  // float4 per step
+  constexpr int NV = A_COUNT * NP / 2;  // 16-byte vectors per step
   constexpr int PER = (NV + NT - 1) / NT;
-  extern __shared__ float4 smem4[];
-  float2* smem = reinterpret_cast<float2*>(smem4);
+  extern __shared__ f4v smem_v[];
+  float2* smem = reinterpret_cast<float2*>(smem_v);
   const float2* cf_l = smem;
   const float2* od_l = smem + A_COUNT * NP;
 
@@ -252,23 +282,22 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   const int n = d.n;
   const int L = d.steps;
   const int row0 = lane * CH;
-  const float4* coef4 = reinterpret_cast<const float4*>(d.coef);
+  const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef);
 
-  // slice-independent off-diagonals
   {
-    const float4* od4 = reinterpret_cast<const float4*>(d.offd);
-    for (int idx = threadIdx.x; idx < NP; idx += NT) smem4[NV + idx] = od4[idx];
+    const f4v* od4 = reinterpret_cast<const f4v*>(d.offd);
+    for (int idx = threadIdx.x; idx < NP; idx += NT) smem_v[NV + idx] = od4[idx];
   }
-  float4 st[PER];
+  f4v st[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int idx = threadIdx.x + k * NT;
-    if (idx < NV) st[k] = coef4[idx];
+    if (NV % NT == 0 || idx < NV) st[k] = coef4[idx];
   }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int idx = threadIdx.x + k * NT;
-    if (idx < NV) smem4[idx] = st[k];
+    if (NV % NT == 0 || idx < NV) smem_v[idx] = st[k];
   }
 
   float2 u[CH];
@@ -277,38 +306,38 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
 #pragma unroll
     for (int m = 0; m < CH; ++m) u[m] = (act && row0 + m < n) ? u0[row0 + m] : make_float2(0.f, 0.f);
   }
-  if (act) store_plane<CH>(d, p, 0, u, row0);
   __syncthreads();
 
   float2 rb[CH];
   for (int s = 1; s <= L; ++s) {
+    if (act) store_plane_fast<CH>(d, p, s - 1, u, row0);
     if (s < L) {
-      const float4* src = coef4 + (size_t)s * NV;
+      const f4v* src = coef4 + (size_t)s * NV;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int idx = threadIdx.x + k * NT;
-        if (idx < NV) st[k] = src[idx];
+        if (NV % NT == 0 || idx < NV) st[k] = src[idx];
       }
     }
-    const float2* rp = nullptr;
-    if (d.rhs != nullptr && act) {
-      const float2* r = (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
+    if (d.rhs != nullptr) {
+      const float2* r = (const float2*)d.rhs + (size_t)(act ? p : 0) * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
-      for (int m = 0; m < CH; ++m) rb[m] = (row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-      rp = rb;
+      for (int m = 0; m < CH; ++m) rb[m] = (act && row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
+      tri_step_lane<CH, true>(u, cf_l, od_l, rb, row0, n, lane);
+    } else {
+      tri_step_lane<CH, false>(u, cf_l, od_l, rb, row0, n, lane);
     }
-    tri_step_lane<CH>(u, cf_l, od_l, rp, row0, n, lane);
-    if (act) store_plane<CH>(d, p, s, u, row0);
     __syncthreads();
     if (s < L) {
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int idx = threadIdx.x + k * NT;
-        if (idx < NV) smem4[idx] = st[k];
+        if (NV % NT == 0 || idx < NV) smem_v[idx] = st[k];
       }
     }
     __syncthreads();
   }
+  if (act) store_plane_fast<CH>(d, p, L, u, row0);
 }
 
 // ------------------------------------------------------------------ plain one-step
@@ -332,18 +361,18 @@ __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s)
     src = (const float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * (s - 1)) * d.out_zstride;
 #pragma unroll
   for (int m = 0; m < CH; ++m) u[m] = (row0 + m < n) ? src[row0 + m] : make_float2(0.f, 0.f);
-  if (s == 1) store_plane<CH>(d, p, 0, u, row0);
+  if (s == 1) store_plane_fast<CH>(d, p, 0, u, row0);
   float2 rb[CH];
-  const float2* rp = nullptr;
+  const float2* cf = (const float2*)d.coef + (size_t)(s - 1) * A_COUNT * NP;
   if (d.rhs != nullptr) {
     const float2* r = (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
     for (int m = 0; m < CH; ++m) rb[m] = (row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-    rp = rb;
+    tri_step_lane<CH, true>(u, cf, (const float2*)d.offd, rb, row0, n, lane);
+  } else {
+    tri_step_lane<CH, false>(u, cf, (const float2*)d.offd, rb, row0, n, lane);
   }
-  const float2* cf = (const float2*)d.coef + (size_t)(s - 1) * A_COUNT * NP;
-  tri_step_lane<CH>(u, cf, (const float2*)d.offd, rp, row0, n, lane);
-  store_plane<CH>(d, p, s, u, row0);
+  store_plane_fast<CH>(d, p, s, u, row0);
 }
 
 // ------------------------------------------------------------------ host side
