@@ -61,44 +61,41 @@ const char* tp_last_error(void);
 /* ------------------------------------------------------------------------
  * Tridiagonal CN stepping (one transverse axis)
  *
- * Rows of the per-slice systems are split into 64*chunk lanes; `chunk` is the
- * number of consecutive rows one GPU lane owns (1, 2, 4, 8, 16 or 32) and must
- * satisfy n <= 64*chunk.  n_pad = 64*chunk.
+ * Every mode solves A_s u_s = B_s u_{s-1} + b_s with tridiagonal A_s and
+ * B_s = diag(sigma_s) - A_s (A0 + B0 = 2I for the CN pair, so sigma = 2 unless
+ * the right-hand operator uses a different slice than the left one), so a step
+ * is u_s = A_s^{-1}(sigma u_{s-1} + b_s) - u_{s-1}.
  *
- * Coefficient buffer (`d_coef`): complex64 [steps][6][n_pad], permuted so that
- * row i is stored at (i % chunk) * 64 + i / chunk.  Its size in bytes is
- * steps * 6 * n_pad * 8 (tp_tri_coef_bytes).
+ * Rows are split into 64 lanes of `chunk` consecutive rows (chunk = 1, 2, 4,
+ * 8, 16 or 32, n <= 64*chunk); n_pad = 64*chunk.  Coefficient buffer
+ * (`d_coef`): complex64 [steps][6][n_pad], row i stored at
+ * (i % chunk) * 64 + i / chunk (tp_tri_coef_bytes gives its size).
  * ------------------------------------------------------------------------ */
 
 size_t tp_tri_coef_bytes(int n, int steps, int chunk);
 
 /* Factor, for every step s = 1..steps, the LHS tridiagonal
- *     lhs(s) = tridiag(lhs_sub, lhs_diag + lhs_sign * c_l(s), lhs_sup)
- * and record the RHS diagonal rhs_diag + rhs_sign * c_r(s), where
- *     c_l(s) = C[:, lhs_col0 + col_step*(s-1)],  c_r(s) = C[:, rhs_col0 + col_step*(s-1)]
- * (a column outside [0, ncol) contributes 0; conj_c conjugates C first).
+ *     A(s) = tridiag(lhs_sub, lhs_diag + lhs_sign * c_l(s), lhs_sup)
+ * where c_l(s) = C[:, lhs_col0 + col_step*(s-1)] (a column outside [0, ncol)
+ * contributes 0; conj_c conjugates C first).  With with_sigma != 0 the
+ * per-row sigma(s) = diag A(s) + rhs_diag + rhs_sign * c_r(s),
+ * c_r(s) = C[:, rhs_col0 + col_step*(s-1)], is stored too; otherwise sigma = 2
+ * is implied.  The caller guarantees rhs_sub = -lhs_sub and rhs_sup = -lhs_sup.
  * d_C: complex128, row-major (n, ncol) = (n, nz-1) exactly as returned by
  *      create_object_contribution(...).reshape(-1, nz-1).
  * d_base: complex128 [6][n] = lhs_sub, lhs_diag, lhs_sup, rhs_sub, rhs_diag, rhs_sup
  *      (sub[0] and sup[n-1] are ignored).
- * The pivots are computed in FP64 (no pivot search: the CN operators are
- * strictly diagonally dominant); products are stored in complex64.
- * Returns TP_ERR_SINGULAR if a pivot is exactly zero (checked on the host
- * only when `d_status` is read back by the caller: d_status[0] != 0). */
+ * One wavefront factors one step: pivots in FP64 (no pivot search: the CN
+ * operators are strictly diagonally dominant), products stored in complex64.
+ * d_status (optional, caller-zeroed int) is set to 1 if a pivot is exactly 0. */
 int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_base,
                   double lhs_sign, double rhs_sign, int conj_c,
                   int lhs_col0, int rhs_col0, int col_step, int steps,
-                  int chunk, void* d_coef, int* d_status, void* stream);
-
-/* RHS off-diagonals in the permuted layout: complex64 [2][n_pad]
- * (sub, sup), rows >= n are zero.  Built by the caller (host) or with
- * tp_tri_offdiag from complex128 arrays. */
-int tp_tri_offdiag(const void* d_base, int n, int chunk, void* d_offd, void* stream);
+                  int chunk, int with_sigma, void* d_coef, int* d_status, void* stream);
 
 /* One scan of a multi-scan sweep. All strides are in complex elements. */
 typedef struct tp_tri_scan {
   const void* coef;       /* d_coef of this scan (step 1 first)                 */
-  const void* offd;       /* d_offd of this scan                                */
   const void* u0;         /* initial planes: complex64, u0 + p*u0_pstride       */
   const void* rhs;        /* optional extra RHS per step, or NULL:
                              rhs + p*rhs_pstride + (s-1)*rhs_sstride, complex64 */
@@ -112,12 +109,14 @@ typedef struct tp_tri_scan {
   int64_t out_pstride, out_zstride, ckpt_pstride, ckpt_zstride, exit_pstride;
   int n, steps, probes;
   int out_z0, out_zdir, ckpt_every;
+  int sigma;              /* coef holds sigma (factored with with_sigma != 0)   */
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe
- * of every scan (nscans <= TP_MAX_SCANS, all with n <= 64*chunk). The
- * per-slice coefficients are staged in LDS and shared by the
- * `probes_per_block` probes (1, 2, 4 or 8) of one workgroup. */
+ * of every scan (nscans <= TP_MAX_SCANS, all with n <= 64*chunk and the same
+ * sigma flag).  One wavefront per probe; the per-step coefficients are staged
+ * in LDS and shared by the `probes_per_block` probes (1, 2, 4 or 8) of one
+ * workgroup. */
 int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int probes_per_block, void* stream);
 
 /* Plain form: advance every probe of every scan by the single step `s`

@@ -40,7 +40,7 @@ def test_abi_version_and_error_path():
     lib = _lib.load()
     assert lib.tp_abi_version() == 1
     # an invalid call fails on the host before touching the GPU
-    rc = lib.tp_tri_factor(None, 10, 5, None, -1.0, 1.0, 0, 0, 0, 1, 5, 3, None, None, None)
+    rc = lib.tp_tri_factor(None, 10, 5, None, -1.0, 1.0, 0, 0, 0, 1, 5, 3, 0, None, None, None)
     assert rc == _lib.TP_ERR_INVALID
     assert b"chunk" in lib.tp_last_error()
     rc = lib.tp_farfield(None, 100, 100, 1, None, None)

@@ -30,7 +30,6 @@ c_size_t = ctypes.c_size_t
 class TriScan(ctypes.Structure):
     _fields_ = [
         ("coef", c_void_p),
-        ("offd", c_void_p),
         ("u0", c_void_p),
         ("rhs", c_void_p),
         ("out", c_void_p),
@@ -50,6 +49,7 @@ class TriScan(ctypes.Structure):
         ("out_z0", c_int),
         ("out_zdir", c_int),
         ("ckpt_every", c_int),
+        ("sigma", c_int),
     ]
 
 
@@ -79,9 +79,8 @@ SIGNATURES = {
     "tp_tri_factor": (
         c_int,
         [c_void_p, c_int, c_int, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int,
-         c_void_p, c_void_p, c_void_p],
+         c_int, c_void_p, c_void_p, c_void_p],
     ),
-    "tp_tri_offdiag": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tp_tri_sweep": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
     "tp_tri_step": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
     "tp_ms_transmission": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p, c_void_p]),

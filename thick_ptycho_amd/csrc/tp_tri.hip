@@ -1,108 +1,183 @@
 // Crank–Nicolson slice stepping with per-slice tridiagonal LU (one transverse axis).
 //
-// Per step s the systems are   lhs(s) u_s = rhs(s) u_{s-1} (+ b_s)   with
-// lhs/rhs tridiagonal and shared by every probe of a scan.  The factor kernel
-// turns lhs(s) into an LU (FP64 pivots, no pivot search: the CN operators are
-// strictly diagonally dominant) and stores, per row i and step s, complex64:
-//   bd   rhs diagonal
-//   l    forward multiplier  l_i = lhs_sub_i / piv_{i-1}      (y_i = r_i - l_i y_{i-1})
-//   gf   prod_{m = chunk start .. i} (-l_m)                   (carry into the chunk)
-//   pinv 1 / piv_i
-//   q    -lhs_sup_i / piv_i                                  (x_i = y_i pinv_i + q_i x_{i+1})
-//   gb   prod_{m = i .. chunk end} q_m                        (carry from the next chunk)
+// Every propagation mode has the form   A_s u_s = B_s u_{s-1} + b_s   with a
+// tridiagonal A_s shared by all probes of a scan and B_s = diag(sigma_s) - A_s
+// (the CN pair A0 = I - K, B0 = I + K sums to 2I, so sigma = 2 except for the
+// exact adjoint, where the RHS uses the next slice's object term).  Hence
+//     u_s = A_s^{-1}(sigma u_{s-1} + b_s) - u_{s-1}
+// and no mat-vec is needed: one tridiagonal solve per probe and step.
 //
-// The sweep assigns one probe to one wavefront; lane l owns the CHUNK
-// consecutive rows [l*CHUNK, (l+1)*CHUNK).  Both triangular recurrences run
-// chunk-locally in registers, the chunk carries are combined by a
-// wavefront-wide scan of affine maps (y -> g*y + c), and a final register pass
-// applies the carries.  No row is ever re-read from HBM within a step.
+// Factor (one wavefront per step, FP64): the pivot recurrence
+// piv_i = d_i - a_i c_{i-1} / piv_{i-1} is a product of 2x2 Moebius matrices
+// [[d_i, -a_i c_{i-1}], [1, 0]]; each lane forms the product over its chunk of
+// rows, an exclusive prefix product across the 64 lanes (normalised, FP64)
+// gives every chunk its incoming pivot, then each lane runs the recurrence over
+// its rows.  Per row and step it stores complex64, in the lane-interleaved
+// layout row i -> (i % CH) * 64 + i / CH:
+//   l    forward multiplier  a_i / piv_{i-1}          (y_i = r_i - l_i y_{i-1})
+//   gf   prod_{m = chunk start .. i} (-l_m)           (carry into the chunk)
+//   pinv 1 / piv_i
+//   q    -c_i / piv_i                                 (x_i = y_i pinv_i + q_i x_{i+1})
+//   gb   prod_{m = i .. chunk end} q_m                (carry from the next chunk)
+//   sg   sigma_i (only when the mode needs it)
+//
+// Sweep: one probe per wavefront, lane l owns rows [l*CH, (l+1)*CH).  Both
+// triangular recurrences run chunk-locally in registers, the chunk carries are
+// combined by wavefront scans of affine maps (DPP row shifts), and a register
+// pass applies them.  The persistent kernel walks every step in one launch
+// with the state in registers; the plain kernel advances one step per launch.
 #include "tp_common.h"
 
 namespace tp {
 
-enum { A_BD = 0, A_L = 1, A_GF = 2, A_PINV = 3, A_Q = 4, A_GB = 5, A_COUNT = 6 };
-
-__host__ __device__ constexpr int perm_index(int i, int chunk) { return (i % chunk) * 64 + i / chunk; }
+enum { A_L = 0, A_GF = 1, A_PINV = 2, A_Q = 3, A_GB = 4, A_SG = 5, A_COUNT = 6 };
 
 // ------------------------------------------------------------------ factor
-template <int CH>
-__global__ __launch_bounds__(64) void tri_factor_kernel(const double2* __restrict__ C, int n, int ncol,
-                                                        const double2* __restrict__ base, double lsign,
-                                                        double rsign, int conj_c, int lcol0, int rcol0,
-                                                        int cstep, int steps, float2* __restrict__ coef,
-                                                        int* status) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= steps) return;
-  constexpr int NP = 64 * CH;
-  float2* o = coef + (size_t)s * A_COUNT * NP;
-  const double2* lsub = base;
-  const double2* ldia = base + n;
-  const double2* lsup = base + 2 * n;
-  const double2* rdia = base + 4 * n;
-  const int lc = lcol0 + cstep * s;
-  const int rc = rcol0 + cstep * s;
-  const bool lok = lc >= 0 && lc < ncol;
-  const bool rok = rc >= 0 && rc < ncol;
-  const double2 zero = make_double2(0.0, 0.0);
-  double2 pinv_prev = zero;
-  bool bad = false;
-  for (int t = 0; t < 64; ++t) {
-    double2 qbuf[CH];
-    double2 gf = make_double2(1.0, 0.0);
-#pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const int i = t * CH + m;
-      const int pi = m * 64 + t;
-      if (i < n) {
-        double2 c = lok ? C[(size_t)i * ncol + lc] : zero;
-        double2 cr = rok ? C[(size_t)i * ncol + rc] : zero;
-        if (conj_c) {
-          c = zconj(c);
-          cr = zconj(cr);
-        }
-        const double2 d = zadd(ldia[i], zscale(c, lsign));
-        const double2 l = (i == 0) ? zero : zmul(lsub[i], pinv_prev);
-        const double2 piv = (i == 0) ? d : zsub(d, zmul(l, lsup[i - 1]));
-        if (piv.x == 0.0 && piv.y == 0.0) bad = true;
-        const double2 pinv = zinv(piv);
-        const double2 q = (i == n - 1) ? zero : zscale(zmul(lsup[i], pinv), -1.0);
-        const double2 bd = zadd(rdia[i], zscale(cr, rsign));
-        gf = zmul(gf, zscale(l, -1.0));
-        o[A_BD * NP + pi] = to_f2(bd);
-        o[A_L * NP + pi] = to_f2(l);
-        o[A_GF * NP + pi] = to_f2(gf);
-        o[A_PINV * NP + pi] = to_f2(pinv);
-        o[A_Q * NP + pi] = to_f2(q);
-        qbuf[m] = q;
-        pinv_prev = pinv;
-      } else {
-#pragma unroll
-        for (int a = 0; a < A_COUNT; ++a) o[a * NP + pi] = make_float2(0.f, 0.f);
-        qbuf[m] = zero;
-      }
-    }
-    double2 gb = make_double2(1.0, 0.0);
-#pragma unroll
-    for (int m = CH - 1; m >= 0; --m) {
-      gb = zmul(gb, qbuf[m]);
-      o[A_GB * NP + m * 64 + t] = to_f2(gb);
-    }
+struct FactorArgs {
+  const double2* C;
+  const double2* base;
+  int n, ncol, steps;
+  double lsign, rsign;
+  int conj_c, lcol0, rcol0, cstep, with_sigma;
+  float2* coef;
+  int* status;
+};
+
+struct Mob {  // 2x2 complex matrix [[a, b], [c, d]]
+  double2 a, b, c, d;
+};
+
+__device__ __forceinline__ Mob mob_mul(const Mob& x, const Mob& y) {  // x * y
+  Mob r;
+  r.a = zadd(zmul(x.a, y.a), zmul(x.b, y.c));
+  r.b = zadd(zmul(x.a, y.b), zmul(x.b, y.d));
+  r.c = zadd(zmul(x.c, y.a), zmul(x.d, y.c));
+  r.d = zadd(zmul(x.c, y.b), zmul(x.d, y.d));
+  return r;
+}
+__device__ __forceinline__ double zabs1(double2 v) { return fabs(v.x) + fabs(v.y); }
+__device__ __forceinline__ Mob mob_normalise(Mob m) {
+  const double s = fmax(fmax(zabs1(m.a), zabs1(m.b)), fmax(zabs1(m.c), zabs1(m.d)));
+  if (s > 0.0) {
+    const double r = 1.0 / s;
+    m.a = zscale(m.a, r);
+    m.b = zscale(m.b, r);
+    m.c = zscale(m.c, r);
+    m.d = zscale(m.d, r);
   }
-  if (bad && status) status[0] = 1;
+  return m;
 }
 
-__global__ void tri_offdiag_kernel(const double2* __restrict__ base, int n, int chunk, float2* __restrict__ offd) {
-  const int np = 64 * chunk;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const int pi = perm_index(i, chunk);
-  float2 e = make_float2(0.f, 0.f), f = make_float2(0.f, 0.f);
-  if (i < n) {
-    if (i > 0) e = to_f2(base[3 * n + i]);
-    if (i < n - 1) f = to_f2(base[5 * n + i]);
+constexpr int FACTOR_WAVES = 4;
+
+template <int CH>
+__global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_kernel(const FactorArgs f) {
+  __shared__ Mob sh[FACTOR_WAVES][2][64];
+  constexpr int NP = 64 * CH;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * FACTOR_WAVES + wave;
+  if (s >= f.steps) return;  // whole wavefront; no workgroup barrier below
+  const int n = f.n;
+  const int row0 = lane * CH;
+  const double2 zero = make_double2(0.0, 0.0);
+  const double2 one = make_double2(1.0, 0.0);
+  const double2* lsub = f.base;
+  const double2* ldia = f.base + n;
+  const double2* lsup = f.base + 2 * n;
+  const double2* rdia = f.base + 4 * n;
+  const int lc = f.lcol0 + f.cstep * s;
+  const int rc = f.rcol0 + f.cstep * s;
+  const bool lok = lc >= 0 && lc < f.ncol;
+  const bool rok = rc >= 0 && rc < f.ncol;
+
+  double2 d[CH], ac[CH];
+#pragma unroll
+  for (int m = 0; m < CH; ++m) {
+    const int i = row0 + m;
+    d[m] = one;
+    ac[m] = zero;
+    if (i < n) {
+      double2 c = lok ? f.C[(size_t)i * f.ncol + lc] : zero;
+      if (f.conj_c) c = zconj(c);
+      d[m] = zadd(ldia[i], zscale(c, f.lsign));
+      if (i > 0) ac[m] = zmul(lsub[i], lsup[i - 1]);
+    }
   }
-  offd[pi] = e;
-  offd[np + pi] = f;
+  // chunk product P = M_last ... M_first, M_i = [[d_i, -a_i c_{i-1}], [1, 0]]
+  Mob P = {one, zero, zero, one};
+#pragma unroll
+  for (int m = 0; m < CH; ++m) {
+    if (row0 + m < n) {
+      Mob M = {d[m], zscale(ac[m], -1.0), one, zero};
+      P = mob_mul(M, P);
+      if ((m & 3) == 3) P = mob_normalise(P);
+    }
+  }
+  P = mob_normalise(P);
+  // inclusive prefix product across lanes (Hillis-Steele through LDS):
+  // Q_l = P_l * P_{l-1} * ... * P_0
+  int buf = 0;
+  sh[wave][0][lane] = P;
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int dist = 1; dist < 64; dist <<= 1) {
+    const Mob own = sh[wave][buf][lane];
+    Mob res = own;
+    if (lane >= dist) res = mob_normalise(mob_mul(own, sh[wave][buf][lane - dist]));
+    __builtin_amdgcn_wave_barrier();
+    sh[wave][buf ^ 1][lane] = res;
+    __builtin_amdgcn_wave_barrier();
+    buf ^= 1;
+  }
+  // state before this chunk: (theta_{start-1}, theta_{start-2}) = Q_{l-1} (1, 0)^T
+  double2 piv_prev = one;
+  if (lane > 0) {
+    const Mob Q = sh[wave][buf][lane - 1];
+    piv_prev = zmul(Q.a, zinv(Q.c));
+  }
+
+  float2* o = f.coef + (size_t)s * A_COUNT * NP;
+  double2 q[CH];
+  double2 gf = one;
+  bool bad = false;
+#pragma unroll
+  for (int m = 0; m < CH; ++m) {
+    const int i = row0 + m;
+    const int pi = m * 64 + lane;
+    if (i < n) {
+      const double2 l = (i == 0) ? zero : zmul(lsub[i], zinv(piv_prev));
+      const double2 piv = (i == 0) ? d[m] : zsub(d[m], zmul(l, lsup[i - 1]));
+      if (piv.x == 0.0 && piv.y == 0.0) bad = true;
+      const double2 pinv = zinv(piv);
+      q[m] = (i == n - 1) ? zero : zscale(zmul(lsup[i], pinv), -1.0);
+      gf = zmul(gf, zscale(l, -1.0));
+      o[A_L * NP + pi] = to_f2(l);
+      o[A_GF * NP + pi] = to_f2(gf);
+      o[A_PINV * NP + pi] = to_f2(pinv);
+      o[A_Q * NP + pi] = to_f2(q[m]);
+      if (f.with_sigma) {
+        double2 cr = rok ? f.C[(size_t)i * f.ncol + rc] : zero;
+        if (f.conj_c) cr = zconj(cr);
+        o[A_SG * NP + pi] = to_f2(zadd(d[m], zadd(rdia[i], zscale(cr, f.rsign))));
+      }
+      piv_prev = piv;
+    } else {
+      q[m] = zero;
+      o[A_L * NP + pi] = make_float2(0.f, 0.f);
+      o[A_GF * NP + pi] = make_float2(0.f, 0.f);
+      o[A_PINV * NP + pi] = make_float2(0.f, 0.f);
+      o[A_Q * NP + pi] = make_float2(0.f, 0.f);
+      if (f.with_sigma) o[A_SG * NP + pi] = make_float2(0.f, 0.f);
+    }
+  }
+  double2 gb = one;
+#pragma unroll
+  for (int m = CH - 1; m >= 0; --m) {
+    gb = zmul(gb, q[m]);
+    o[A_GB * NP + m * 64 + lane] = to_f2(gb);
+  }
+  if (bad && f.status) f.status[0] = 1;
 }
 
 // ------------------------------------------------------------------ wave scans
@@ -133,8 +208,7 @@ constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, 
 constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
 constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
-__device__ __forceinline__ void scan_affine_up(float2& A, float2& B, int lane) {
-  (void)lane;
+__device__ __forceinline__ void scan_affine_up(float2& A, float2& B) {
   affine_step<DPP_ROW_SHR1>(A, B);
   affine_step<DPP_ROW_SHR2>(A, B);
   affine_step<DPP_ROW_SHR4>(A, B);
@@ -172,33 +246,34 @@ __device__ __forceinline__ void scan_affine_down(float2& A, float2& B, int lane)
 __device__ __forceinline__ float2 from_prev_lane(float2 v) { return dpp_c<DPP_WAVE_SHR1>(make_float2(0.f, 0.f), v); }
 __device__ __forceinline__ float2 from_next_lane(float2 v) { return dpp_c<DPP_WAVE_SHL1>(make_float2(0.f, 0.f), v); }
 
-// One CN step for the CH rows of this lane.  `cf` points at the step's six
-// permuted coefficient arrays, `od` at the two permuted off-diagonal arrays
-// (LDS or global).  `rhs` (natural order rows of this lane) may be null.
-template <int CH, bool HAS_RHS, typename Ptr>
-__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, const float2 (&rhs)[CH], int row0,
-                                              int n, int lane) {
+// One CN step for the CH rows of this lane:  u <- A^{-1}(sigma u + b) - u.
+// `cf` points at the step's coefficient arrays (LDS or global, lane-interleaved).
+// Without SIGMA (sigma = 2):  r = u + b/2 and u <- 2 x - u.
+// With SIGMA:                 r = sigma u + b and u <- x - u.
+struct NoHook {
+  __device__ __forceinline__ void operator()(int) const {}
+};
+
+template <int CH, bool HAS_RHS, bool SIGMA, typename Ptr, typename Hook = NoHook>
+__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
+                                              const Hook& hook = Hook()) {
   constexpr int NP = 64 * CH;
-  const float2 left = from_prev_lane(u[CH - 1]);
-  const float2 right = from_next_lane(u[0]);
   float2 y[CH];
 #pragma unroll
   for (int m = 0; m < CH; ++m) {
-    const int k = m * 64 + lane;
-    const float2 um = (m == 0) ? left : u[m - 1];
-    const float2 up = (m == CH - 1) ? right : u[m + 1];
-    float2 r = cmul(cf[A_BD * NP + k], u[m]);
-    r = cfma(od[k], um, r);
-    r = cfma(od[NP + k], up, r);
-    if (HAS_RHS) r = cadd(r, rhs[m]);
+    float2 r = u[m];
+    if (SIGMA) r = cmul(cf[A_SG * NP + m * 64 + lane], r);
+    if (HAS_RHS) r = SIGMA ? cadd(r, rhs[m]) : make_float2(fmaf(0.5f, rhs[m].x, r.x), fmaf(0.5f, rhs[m].y, r.y));
     y[m] = r;
   }
+  hook(0);
 #pragma unroll
   for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
+  hook(1);
   {
     float2 A = cf[A_GF * NP + (CH - 1) * 64 + lane];
     float2 B = y[CH - 1];
-    scan_affine_up(A, B, lane);
+    scan_affine_up(A, B);
     const float2 yin = from_prev_lane(B);
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
@@ -207,14 +282,19 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, Ptr od, c
   for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
 #pragma unroll
   for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
+  hook(2);
   {
     float2 A = cf[A_GB * NP + lane];
     float2 B = y[0];
     scan_affine_down(A, B, lane);
     const float2 xin = from_next_lane(B);
 #pragma unroll
-    for (int m = 0; m < CH; ++m) u[m] = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
+    for (int m = 0; m < CH; ++m) {
+      const float2 x = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
+      u[m] = SIGMA ? csub(x, u[m]) : make_float2(fmaf(2.f, x.x, -u[m].x), fmaf(2.f, x.y, -u[m].y));
+    }
   }
+  hook(3);
 }
 
 struct SweepArgs {
@@ -222,6 +302,7 @@ struct SweepArgs {
   int blk_start[TP_MAX_SCANS + 1];
   int nscans;
   int ppb;
+  int nblk;
 };
 
 __device__ __forceinline__ int find_scan(const SweepArgs& a, int blk) {
@@ -232,8 +313,14 @@ __device__ __forceinline__ int find_scan(const SweepArgs& a, int blk) {
   return sc;
 }
 
-// ------------------------------------------------------------------ persistent sweep
-typedef float f4v __attribute__((ext_vector_type(4)));
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so that the probes of one scan (which share
+// every step's coefficients) meet in as few L2 caches as possible.
+__device__ __forceinline__ int xcd_block(int hw, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = hw & 7, j = hw >> 3;
+  return x * q + min(x, r) + j;
+}
 
 template <int CH>
 __device__ __forceinline__ void store_rows(float2* o, const float2 (&u)[CH], int row0, int n) {
@@ -248,31 +335,100 @@ __device__ __forceinline__ void store_rows(float2* o, const float2 (&u)[CH], int
 }
 
 template <int CH>
-__device__ __forceinline__ void store_plane_fast(const tp_tri_scan& d, int p, int s, const float2 (&u)[CH], int row0) {
+__device__ __forceinline__ void store_plane(const tp_tri_scan& d, int p, int s, const float2 (&u)[CH], int row0) {
   const int n = d.n;
-  if (d.out) store_rows<CH>((float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride, u, row0, n);
+  if (d.out)
+    store_rows<CH>((float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride,
+                   u, row0, n);
   if (d.ckpt && (s % d.ckpt_every) == 0)
-    store_rows<CH>((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, u, row0, n);
+    store_rows<CH>((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, u,
+                   row0, n);
   if (d.exitp && s == d.steps) store_rows<CH>((float2*)d.exitp + (size_t)p * d.exit_pstride, u, row0, n);
 }
 
+// Coalesced plane stores: the wavefront's plane goes through an LDS tile
+// (row i at i + 2*(i/16): two pad words per 16 rows against bank conflicts,
+// row pairs stay 16-byte aligned) and leaves as 16-byte stores, each
+// instruction writing 64 consecutive row pairs (1 KiB).  The stores are split
+// into four groups that the step interleaves with its compute phases.
+__device__ __forceinline__ int tile_index(int i) { return i + 2 * (i >> 4); }
+
+template <int CH>
+__device__ __forceinline__ void tile_put(float2* tile, const float2 (&u)[CH], int row0) {
+#pragma unroll
+  for (int m = 0; m < CH; ++m) tile[tile_index(row0 + m)] = u[m];
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int CH>
+struct PlaneWriter {
+  float2* dst[3];
+  const float2* tile;
+  int n, lane;
+
+  __device__ __forceinline__ void init(const tp_tri_scan& d, int p, int s, bool act, const float2* t, int ln) {
+    tile = t;
+    n = d.n;
+    lane = ln;
+    dst[0] = (act && d.out)
+                 ? (float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride
+                 : nullptr;
+    dst[1] = (act && d.ckpt && (s % d.ckpt_every) == 0)
+                 ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride
+                 : nullptr;
+    dst[2] = (act && d.exitp && s == d.steps) ? (float2*)d.exitp + (size_t)p * d.exit_pstride : nullptr;
+  }
+
+  // group g of 4: row pairs q = lane + 64 k for k in [g*CH/8, (g+1)*CH/8)
+  __device__ __forceinline__ void operator()(int g) const {
+    constexpr int KP = CH / 2 > 0 ? CH / 2 : 1;  // pair-instructions per plane
+    const int k0 = (g * KP) / 4, k1 = ((g + 1) * KP) / 4;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      float2* o = dst[t];
+      if (o == nullptr) continue;
+      const bool aligned = (reinterpret_cast<uintptr_t>(o) & 15) == 0;
+      for (int k = k0; k < k1; ++k) {
+        const int j = 2 * (lane + 64 * k);
+        if (aligned && j + 1 < n) {
+          typedef float f4 __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<f4*>(o + j) = *reinterpret_cast<const f4*>(tile + tile_index(j));
+        } else {
+          if (j < n) o[j] = tile[tile_index(j)];
+          if (j + 1 < n) o[j + 1] = tile[tile_index(j + 1)];
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void all() const {
+    (*this)(0);
+    (*this)(1);
+    (*this)(2);
+    (*this)(3);
+  }
+};
+
+// ------------------------------------------------------------------ persistent sweep
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 // One workgroup = PPB wavefronts = PPB probes of one scan walking all steps in
-// lockstep.  The step's coefficients (6 * 64*CH complex64) are staged in LDS
-// and shared by the PPB wavefronts; the next step's block is fetched into
-// registers while the current step computes.  The plane computed in step s is
-// written at the start of step s+1 so that the stores drain behind the math.
-template <int CH, int PPB>
+// lockstep.  The step's coefficients are staged in LDS and shared by the PPB
+// wavefronts; the next step's block is fetched into registers while the
+// current step computes, and the plane computed in step s is written at the
+// start of step s+1 (before that fetch) so the stores drain behind the math.
+template <int CH, int PPB, bool SIGMA>
 __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
-  constexpr int NV = A_COUNT * NP / 2;  // 16-byte vectors per step
+  constexpr int NA = SIGMA ? 6 : 5;
+  constexpr int NV = NA * NP / 2;             // 16-byte vectors staged per step
+  constexpr int STRIDE_V = A_COUNT * NP / 2;  // 16-byte vectors per step in memory
   constexpr int PER = (NV + NT - 1) / NT;
   extern __shared__ f4v smem_v[];
-  float2* smem = reinterpret_cast<float2*>(smem_v);
-  const float2* cf_l = smem;
-  const float2* od_l = smem + A_COUNT * NP;
+  const float2* cf_l = reinterpret_cast<const float2*>(smem_v);
+  constexpr int TILE = NP + NP / 8;  // per-wavefront store tile (float2)
 
-  const int blk = blockIdx.x;
+  const int blk = xcd_block(blockIdx.x, a.nblk);
   const int sc = find_scan(a, blk);
   const tp_tri_scan& d = a.scans[sc];
   const int wave = threadIdx.x >> 6;
@@ -283,11 +439,8 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   const int L = d.steps;
   const int row0 = lane * CH;
   const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef);
+  float2* tile = reinterpret_cast<float2*>(smem_v) + NA * NP + wave * TILE;
 
-  {
-    const f4v* od4 = reinterpret_cast<const f4v*>(d.offd);
-    for (int idx = threadIdx.x; idx < NP; idx += NT) smem_v[NV + idx] = od4[idx];
-  }
   f4v st[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -300,19 +453,37 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
     if (NV % NT == 0 || idx < NV) smem_v[idx] = st[k];
   }
 
+  // The initial plane arrives through the LDS tile: coalesced, and the state
+  // registers never hold a pending global load when the step loop starts
+  // (otherwise the first use of u in every iteration waits on vmcnt(0)).
   float2 u[CH];
   {
     const float2* u0 = (const float2*)d.u0 + (size_t)(act ? p : 0) * d.u0_pstride;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) u[m] = (act && row0 + m < n) ? u0[row0 + m] : make_float2(0.f, 0.f);
+    for (int k = 0; k < CH; ++k) {
+      const int j = lane + 64 * k;
+      tile[tile_index(j)] = (act && j < n) ? u0[j] : make_float2(0.f, 0.f);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = 0; m < CH; ++m) u[m] = tile[tile_index(row0 + m)];
+    __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
 
   float2 rb[CH];
   for (int s = 1; s <= L; ++s) {
-    if (act) store_plane_fast<CH>(d, p, s - 1, u, row0);
+    // Plane s-1 is written first, then the next step's coefficients are
+    // fetched: vmcnt retires in order, so the wait for that fetch at the end
+    // of the step only covers stores that have had a whole step to drain.
+    {
+      tile_put<CH>(tile, u, row0);
+      PlaneWriter<CH> w;
+      w.init(d, p, s - 1, act, tile, lane);
+      w.all();
+    }
     if (s < L) {
-      const f4v* src = coef4 + (size_t)s * NV;
+      const f4v* src = coef4 + (size_t)s * STRIDE_V;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int idx = threadIdx.x + k * NT;
@@ -323,9 +494,9 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
       const float2* r = (const float2*)d.rhs + (size_t)(act ? p : 0) * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
       for (int m = 0; m < CH; ++m) rb[m] = (act && row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-      tri_step_lane<CH, true>(u, cf_l, od_l, rb, row0, n, lane);
+      tri_step_lane<CH, true, SIGMA>(u, cf_l, rb, lane);
     } else {
-      tri_step_lane<CH, false>(u, cf_l, od_l, rb, row0, n, lane);
+      tri_step_lane<CH, false, SIGMA>(u, cf_l, rb, lane);
     }
     __syncthreads();
     if (s < L) {
@@ -337,42 +508,80 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
     }
     __syncthreads();
   }
-  if (act) store_plane_fast<CH>(d, p, L, u, row0);
+  tile_put<CH>(tile, u, row0);
+  {
+    PlaneWriter<CH> w;
+    w.init(d, p, L, act, tile, lane);
+    w.all();
+  }
 }
 
 // ------------------------------------------------------------------ plain one-step
-template <int CH>
+// One launch per step; plane s-1 is read from memory and plane s written, both
+// through the per-wavefront LDS tile so that global accesses are coalesced.
+template <int CH, bool SIGMA>
 __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s) {
   constexpr int NP = 64 * CH;
-  const int blk = blockIdx.x;
+  constexpr int TILE = NP + NP / 8;
+  extern __shared__ float2 tiles[];
+  const int blk = xcd_block(blockIdx.x, a.nblk);
   const int sc = find_scan(a, blk);
   const tp_tri_scan& d = a.scans[sc];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int p = (blk - a.blk_start[sc]) * a.ppb + wave;
-  if (wave >= a.ppb || p >= d.probes) return;
+  if (wave >= a.ppb || p >= d.probes) return;  // whole wavefront; no workgroup barrier below
   const int n = d.n;
   const int row0 = lane * CH;
-  float2 u[CH];
+  float2* tile = tiles + wave * TILE;
   const float2* src;
   if (s == 1)
     src = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
   else
     src = (const float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * (s - 1)) * d.out_zstride;
 #pragma unroll
-  for (int m = 0; m < CH; ++m) u[m] = (row0 + m < n) ? src[row0 + m] : make_float2(0.f, 0.f);
-  if (s == 1) store_plane_fast<CH>(d, p, 0, u, row0);
+  for (int k = 0; k < CH; ++k) {
+    const int j = lane + 64 * k;
+    tile[tile_index(j)] = (j < n) ? src[j] : make_float2(0.f, 0.f);
+  }
+  __builtin_amdgcn_wave_barrier();
+  float2 u[CH];
+#pragma unroll
+  for (int m = 0; m < CH; ++m) u[m] = tile[tile_index(row0 + m)];
+  __builtin_amdgcn_wave_barrier();
+  if (s == 1) {
+    PlaneWriter<CH> w0;
+    w0.init(d, p, 0, true, tile, lane);
+    w0.all();
+    __builtin_amdgcn_wave_barrier();
+  }
   float2 rb[CH];
   const float2* cf = (const float2*)d.coef + (size_t)(s - 1) * A_COUNT * NP;
   if (d.rhs != nullptr) {
     const float2* r = (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
     for (int m = 0; m < CH; ++m) rb[m] = (row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-    tri_step_lane<CH, true>(u, cf, (const float2*)d.offd, rb, row0, n, lane);
+    tri_step_lane<CH, true, SIGMA>(u, cf, rb, lane);
   } else {
-    tri_step_lane<CH, false>(u, cf, (const float2*)d.offd, rb, row0, n, lane);
+    tri_step_lane<CH, false, SIGMA>(u, cf, rb, lane);
   }
-  store_plane_fast<CH>(d, p, s, u, row0);
+  tile_put<CH>(tile, u, row0);
+  PlaneWriter<CH> w;
+  w.init(d, p, s, true, tile, lane);
+  w.all();
+}
+
+template <int CH, bool SIGMA>
+static int launch_step(const SweepArgs& a, int s, hipStream_t st) {
+  constexpr int NP = 64 * CH;
+  const size_t lds = (size_t)4 * (NP + NP / 8) * sizeof(float2);
+  if (lds > 65536) {
+    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_step_kernel<CH, SIGMA>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  hipLaunchKernelGGL((tri_step_kernel<CH, SIGMA>), dim3(a.nblk), dim3(256), lds, st, a, s);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
 }
 
 // ------------------------------------------------------------------ host side
@@ -385,29 +594,34 @@ static int check_chunk(int chunk) {
   }
 }
 
-static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, SweepArgs& a, int& nblk) {
+static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, SweepArgs& a, int& sigma) {
   if (nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
     return TP_ERR_INVALID;
   }
   memset(&a, 0, sizeof(a));
-  nblk = 0;
+  int nblk = 0;
+  sigma = scans[0].sigma;
   for (int i = 0; i < nscans; ++i) {
     const tp_tri_scan& d = scans[i];
     if (d.n < 1 || d.n > 64 * chunk) {
       set_error("scan %d: n = %d outside [1, 64*chunk = %d]", i, d.n, 64 * chunk);
       return TP_ERR_INVALID;
     }
-    if (d.steps < 1 || d.probes < 0 || !d.coef || !d.offd || !d.u0) {
-      set_error("scan %d: steps >= 1, probes >= 0, coef, offd and u0 are required", i);
+    if (d.steps < 1 || d.probes < 0 || !d.coef || !d.u0) {
+      set_error("scan %d: steps >= 1, probes >= 0, coef and u0 are required", i);
+      return TP_ERR_INVALID;
+    }
+    if (d.sigma != sigma) {
+      set_error("scan %d: all scans of one launch must share the sigma flag", i);
       return TP_ERR_INVALID;
     }
     if (d.ckpt && d.ckpt_every < 1) {
       set_error("scan %d: ckpt_every must be >= 1 when ckpt is set", i);
       return TP_ERR_INVALID;
     }
-    if ((reinterpret_cast<uintptr_t>(d.coef) & 15) || (reinterpret_cast<uintptr_t>(d.offd) & 15)) {
-      set_error("scan %d: coef and offd must be 16-byte aligned", i);
+    if (reinterpret_cast<uintptr_t>(d.coef) & 15) {
+      set_error("scan %d: coef must be 16-byte aligned", i);
       return TP_ERR_INVALID;
     }
     a.scans[i] = d;
@@ -417,33 +631,44 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
   a.blk_start[nscans] = nblk;
   a.nscans = nscans;
   a.ppb = ppb;
+  a.nblk = nblk;
   return TP_OK;
 }
 
-template <int CH, int PPB>
-static int launch_sweep(const SweepArgs& a, int nblk, hipStream_t st) {
+template <int CH, int PPB, bool SIGMA>
+static int launch_sweep(const SweepArgs& a, hipStream_t st) {
   constexpr int NP = 64 * CH;
-  const size_t lds = (size_t)(A_COUNT + 2) * NP * sizeof(float2);
+  const size_t lds = ((size_t)(SIGMA ? 6 : 5) * NP + (size_t)PPB * (NP + NP / 8)) * sizeof(float2);
+  if (lds > 163840) {
+    set_error("chunk %d with %d probes per block needs %zu bytes of LDS (> 160 KiB); use fewer probes per block",
+              CH, PPB, lds);
+    return TP_ERR_INVALID;
+  }
   if (lds > 65536) {
-    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB>,
+    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB>), dim3(nblk), dim3(64 * PPB), lds, st, a);
+  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
 
-template <int CH>
-static int sweep_ppb(const SweepArgs& a, int nblk, int ppb, hipStream_t st) {
+template <int CH, bool SIGMA>
+static int sweep_ppb(const SweepArgs& a, int ppb, hipStream_t st) {
   switch (ppb) {
-    case 1: return launch_sweep<CH, 1>(a, nblk, st);
-    case 2: return launch_sweep<CH, 2>(a, nblk, st);
-    case 4: return launch_sweep<CH, 4>(a, nblk, st);
-    case 8: return launch_sweep<CH, 8>(a, nblk, st);
+    case 1: return launch_sweep<CH, 1, SIGMA>(a, st);
+    case 2: return launch_sweep<CH, 2, SIGMA>(a, st);
+    case 4: return launch_sweep<CH, 4, SIGMA>(a, st);
+    case 8: return launch_sweep<CH, 8, SIGMA>(a, st);
     default:
       set_error("probes_per_block must be 1, 2, 4 or 8 (got %d)", ppb);
       return TP_ERR_INVALID;
   }
+}
+
+template <int CH>
+static int sweep_sigma(const SweepArgs& a, int ppb, int sigma, hipStream_t st) {
+  return sigma ? sweep_ppb<CH, true>(a, ppb, st) : sweep_ppb<CH, false>(a, ppb, st);
 }
 
 }  // namespace tp
@@ -456,8 +681,8 @@ extern "C" size_t tp_tri_coef_bytes(int n, int steps, int chunk) {
 }
 
 extern "C" int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_base, double lhs_sign, double rhs_sign,
-                             int conj_c, int lhs_col0, int rhs_col0, int col_step, int steps, int chunk, void* d_coef,
-                             int* d_status, void* stream) {
+                             int conj_c, int lhs_col0, int rhs_col0, int col_step, int steps, int chunk,
+                             int with_sigma, void* d_coef, int* d_status, void* stream) {
   int rc = check_chunk(chunk);
   if (rc) return rc;
   if (n < 1 || n > 64 * chunk || steps < 1 || ncol < 1 || !d_C || !d_base || !d_coef) {
@@ -465,16 +690,25 @@ extern "C" int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_bas
               ncol);
     return TP_ERR_INVALID;
   }
+  FactorArgs f;
+  f.C = (const double2*)d_C;
+  f.base = (const double2*)d_base;
+  f.n = n;
+  f.ncol = ncol;
+  f.steps = steps;
+  f.lsign = lhs_sign;
+  f.rsign = rhs_sign;
+  f.conj_c = conj_c;
+  f.lcol0 = lhs_col0;
+  f.rcol0 = rhs_col0;
+  f.cstep = col_step;
+  f.with_sigma = with_sigma;
+  f.coef = (float2*)d_coef;
+  f.status = d_status;
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((steps + 63) / 64), block(64);
-  const double2* C = (const double2*)d_C;
-  const double2* B = (const double2*)d_base;
-  float2* o = (float2*)d_coef;
-#define TP_FACTOR_CASE(CH)                                                                                  \
-  case CH:                                                                                                  \
-    hipLaunchKernelGGL((tri_factor_kernel<CH>), grid, block, 0, st, C, n, ncol, B, lhs_sign, rhs_sign, conj_c, \
-                       lhs_col0, rhs_col0, col_step, steps, o, d_status);                                   \
-    break;
+  const dim3 grid((steps + FACTOR_WAVES - 1) / FACTOR_WAVES), block(64 * FACTOR_WAVES);
+#define TP_FACTOR_CASE(CH) \
+  case CH: hipLaunchKernelGGL((tri_factor_kernel<CH>), grid, block, 0, st, f); break;
   switch (chunk) {
     TP_FACTOR_CASE(1)
     TP_FACTOR_CASE(2)
@@ -488,36 +722,22 @@ extern "C" int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_bas
   return TP_OK;
 }
 
-extern "C" int tp_tri_offdiag(const void* d_base, int n, int chunk, void* d_offd, void* stream) {
-  int rc = check_chunk(chunk);
-  if (rc) return rc;
-  if (n < 1 || n > 64 * chunk || !d_base || !d_offd) {
-    set_error("tp_tri_offdiag: invalid arguments (n=%d, chunk=%d)", n, chunk);
-    return TP_ERR_INVALID;
-  }
-  const int np = 64 * chunk;
-  hipLaunchKernelGGL(tri_offdiag_kernel, dim3((np + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                     (const double2*)d_base, n, chunk, (float2*)d_offd);
-  TP_CHECK_HIP(hipGetLastError());
-  return TP_OK;
-}
-
 extern "C" int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int probes_per_block, void* stream) {
   int rc = check_chunk(chunk);
   if (rc) return rc;
   SweepArgs a;
-  int nblk = 0;
-  rc = build_args(scans, nscans, chunk, probes_per_block, a, nblk);
+  int sigma = 0;
+  rc = build_args(scans, nscans, chunk, probes_per_block, a, sigma);
   if (rc) return rc;
-  if (nblk == 0) return TP_OK;
+  if (a.nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
   switch (chunk) {
-    case 1: return sweep_ppb<1>(a, nblk, probes_per_block, st);
-    case 2: return sweep_ppb<2>(a, nblk, probes_per_block, st);
-    case 4: return sweep_ppb<4>(a, nblk, probes_per_block, st);
-    case 8: return sweep_ppb<8>(a, nblk, probes_per_block, st);
-    case 16: return sweep_ppb<16>(a, nblk, probes_per_block, st);
-    case 32: return sweep_ppb<32>(a, nblk, probes_per_block, st);
+    case 1: return sweep_sigma<1>(a, probes_per_block, sigma, st);
+    case 2: return sweep_sigma<2>(a, probes_per_block, sigma, st);
+    case 4: return sweep_sigma<4>(a, probes_per_block, sigma, st);
+    case 8: return sweep_sigma<8>(a, probes_per_block, sigma, st);
+    case 16: return sweep_sigma<16>(a, probes_per_block, sigma, st);
+    case 32: return sweep_sigma<32>(a, probes_per_block, sigma, st);
   }
   return TP_ERR_INVALID;
 }
@@ -526,9 +746,9 @@ extern "C" int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int 
   int rc = check_chunk(chunk);
   if (rc) return rc;
   SweepArgs a;
-  int nblk = 0;
+  int sigma = 0;
   const int ppb = 4;
-  rc = build_args(scans, nscans, chunk, ppb, a, nblk);
+  rc = build_args(scans, nscans, chunk, ppb, a, sigma);
   if (rc) return rc;
   for (int i = 0; i < nscans; ++i) {
     if (!scans[i].out || s < 1 || s > scans[i].steps) {
@@ -536,12 +756,10 @@ extern "C" int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int 
       return TP_ERR_INVALID;
     }
   }
-  if (nblk == 0) return TP_OK;
+  if (a.nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
-#define TP_STEP_CASE(CH)                                                                  \
-  case CH:                                                                                \
-    hipLaunchKernelGGL((tri_step_kernel<CH>), dim3(nblk), dim3(64 * ppb), 0, st, a, s); \
-    break;
+#define TP_STEP_CASE(CH) \
+  case CH: return sigma ? launch_step<CH, true>(a, s, st) : launch_step<CH, false>(a, s, st);
   switch (chunk) {
     TP_STEP_CASE(1)
     TP_STEP_CASE(2)
@@ -551,6 +769,5 @@ extern "C" int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int 
     TP_STEP_CASE(32)
   }
 #undef TP_STEP_CASE
-  TP_CHECK_HIP(hipGetLastError());
-  return TP_OK;
+  return TP_ERR_INVALID;
 }

@@ -121,6 +121,16 @@ class ModePlan:
     rhs_col0: int
     col_step: int
     b_sign: float = 1.0
+    sigma: bool = False  # rhs diagonal + lhs diagonal differs from 2 (different slices)
+
+    def check_complementary(self) -> None:
+        """The kernels use rhs = diag(sigma) - lhs: the off-diagonals must cancel."""
+        if not (np.allclose(self.rhs.sub[1:], -self.lhs.sub[1:], rtol=0, atol=1e-14 * (1 + np.abs(self.lhs.sub).max()))
+                and np.allclose(self.rhs.sup[:-1], -self.lhs.sup[:-1], rtol=0,
+                                atol=1e-14 * (1 + np.abs(self.lhs.sup).max()))):
+            raise ValueError("rhs off-diagonals are not the negated lhs off-diagonals")
+        if not self.sigma and not np.allclose(self.lhs.diag + self.rhs.diag, 2.0, rtol=0, atol=1e-12):
+            raise ValueError("lhs + rhs diagonals differ from 2; factor with sigma")
 
     def base_array(self) -> np.ndarray:
         """complex128 [6][N]: lhs sub/diag/sup, rhs sub/diag/sup."""
@@ -147,7 +157,7 @@ def mode_plan(A: Tridiag, B: Tridiag, mode: str, L: int) -> ModePlan:
     if mode == "reverse":
         return ModePlan(B, A, 1.0, -1.0, 0, L - 1, L - 1, -1, b_sign=-1.0)
     if mode == "exact_adjoint":
-        return ModePlan(A.conj_transpose(), B.conj_transpose(), -1.0, 1.0, 1, L - 1, L, -1)
+        return ModePlan(A.conj_transpose(), B.conj_transpose(), -1.0, 1.0, 1, L - 1, L, -1, sigma=True)
     raise AssertionError(f"Invalid mode: {mode!r}")
 
 

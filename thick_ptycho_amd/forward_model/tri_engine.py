@@ -23,13 +23,13 @@ from .operators import ModePlan, Tridiag, chunk_for, mode_plan
 @dataclass
 class TriFactors:
     coef: torch.Tensor
-    offd: torch.Tensor
     n: int
     steps: int
     chunk: int
     mode: str
     b_sign: float
     status: torch.Tensor
+    sigma: bool
     keep: list = field(default_factory=list)
 
     def check(self) -> None:
@@ -48,21 +48,20 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
     C = C.contiguous()
     chunk = chunk or chunk_for(n)
     plan: ModePlan = mode_plan(A, B, mode, L)
+    plan.check_complementary()
     dev = C.device
     base = torch.from_numpy(plan.base_array().astype(np.complex128)).to(dev)
     coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=dev)
-    offd = torch.empty(2 * 64 * chunk, dtype=torch.complex64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     lib = _lib.load()
     sh = _lib.stream_handle(stream)
     _lib.check(
         lib.tp_tri_factor(_lib.ptr(C), n, L, _lib.ptr(base), plan.lhs_sign, plan.rhs_sign, plan.conj_c,
-                          plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, _lib.ptr(coef),
+                          plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, int(plan.sigma), _lib.ptr(coef),
                           _lib.ptr(status), sh),
         "tp_tri_factor",
     )
-    _lib.check(lib.tp_tri_offdiag(_lib.ptr(base), n, chunk, _lib.ptr(offd), sh), "tp_tri_offdiag")
-    return TriFactors(coef, offd, n, L, chunk, mode, plan.b_sign, status, keep=[base, C])
+    return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C])
 
 
 @dataclass
@@ -92,7 +91,7 @@ class TriScanSpec:
         n, L = f.n, f.steps
         s = _lib.TriScan()
         s.coef = _lib.ptr(f.coef)
-        s.offd = _lib.ptr(f.offd)
+        s.sigma = int(f.sigma)
         assert self.u0.dtype == torch.complex64 and self.u0.shape[-1] == n and self.u0.stride(-1) == 1
         s.u0 = _lib.ptr(self.u0)
         s.u0_pstride = self.u0.stride(0) if P > 1 else n
@@ -138,6 +137,8 @@ def _structs(specs: Sequence[TriScanSpec]):
 def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) -> None:
     """Persistent batched sweep over all steps of all scans (one launch)."""
     arr, chunk = _structs(specs)
+    if chunk >= 32:
+        probes_per_block = min(probes_per_block, 2)  # LDS: 2048-row blocks
     lib = _lib.load()
     _lib.check(lib.tp_tri_sweep(arr, len(specs), chunk, probes_per_block, _lib.stream_handle(stream)), "tp_tri_sweep")
 
