@@ -29,6 +29,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the persistent sweep on this workload (nz = 1024,
+# 4 probes per workgroup), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a
+# 1 GiB copy in the same process (profiles/tri_sweep_batched_cfg1_traffic.md).
+MEASURED_SWEEP_TRAFFIC = {(1024, 4): 15.23e9}
 
 
 def log(msg: str) -> None:
@@ -214,7 +218,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_SWEEP_TRAFFIC.get((args.nz, args.ppb)),
             "launch_ms": t_kernel * 1e3,
             "algorithmic_bytes": alg_bytes,
         },
