@@ -23,9 +23,14 @@
  *                      base_solver.py:84-154): u_j = LU_j^{-1}(B_j u_{j-1} + b).
  *   tp_tri_step     <- one iteration j of the loop at iterative_lu.py:185-193
  *                      for every probe (the "plain" one-launch-per-slice form).
- *   tp_tri_gradient <- the adjoint sweep plus gradient accumulation of
+ *   tp_tri_sweep with gfwd/grad set
+ *                   <- the adjoint sweep plus gradient accumulation of
  *                      ReconstructorPWE.compute_grad_least_squares
  *                      (thick_ptycho/reconstruction/pwe_reconstructor.py:100-164).
+ *   tp_amplitude_seed
+ *                   <- ReconstructorBase.get_error_in_exit_wave /
+ *                      _apply_phase_retrieval_constraint
+ *                      (thick_ptycho/reconstruction/base_reconstructor.py:87-207).
  *   tp_ms_transmission, tp_ms_sweep
  *                   <- MSForwardModelSolver._solve_single_probe
  *                      (thick_ptycho/forward_model/multislice/ms_solver.py:90-159)
@@ -110,6 +115,18 @@ typedef struct tp_tri_scan {
   int n, steps, probes;
   int out_z0, out_zdir, ckpt_every;
   int sigma;              /* coef holds sigma (factored with with_sigma != 0)   */
+  int rhs_steps;          /* rhs applies to steps 1..rhs_steps (0 = every step) */
+  /* Gradient accumulation (persistent sweep only).  After step s the new state
+   * lambda is combined with two planes of `gfwd`, a = plane gfwd_z0 + gfwd_zdir*(s-1)
+   * and b = plane gfwd_z0 + gfwd_zdir*s, and
+   *     grad[row] += sum over probes of conj(lambda) * (a + b),
+   *     row = grad_z0 + grad_zdir*(s-1)
+   * (b is left out on row 0 when grad_skip_row0 != 0).  grad: complex64 rows of
+   * n values, row stride grad_zstride; accumulated with atomics (caller zeroes). */
+  const void* gfwd;
+  void* grad;
+  int64_t gfwd_pstride, gfwd_zstride, grad_zstride;
+  int gfwd_z0, gfwd_zdir, grad_z0, grad_zdir, grad_skip_row0;
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe
@@ -153,6 +170,15 @@ int tp_ms_sweep(const tp_ms_scan* scans, int nscans, void* stream);
 /* Plain form: one slice z (0-based) for every probe, reading plane z from
  * `out` (u0 when z == 0, also writing plane 0) and writing plane z+1. */
 int tp_ms_step(const tp_ms_scan* scans, int nscans, int z, void* stream);
+
+/* Amplitude-misfit seed for the adjoint: U = fft(exit) per probe,
+ *   loss[p]  = sum_k (|U_k| - a_k)^2,
+ *   seed[p]  = scale * sum_k W_k exp(+2 pi i k x / n),  W = (|U| - a) U / |U| (0 where U = 0),
+ * i.e. scale = 1 gives dJ/d conj(exit) and scale = 1/n the reference's exit-wave
+ * error exit - ifft(a U/|U|).  exit, seed: complex64 [probes][n]; a: float32
+ * [probes][n] (sqrt of the measured intensities); loss: float32 [probes]. */
+int tp_amplitude_seed(const void* d_exit, const void* d_sqrt_i, int n, int probes, double scale, void* d_seed,
+                      void* d_loss, void* stream);
 
 /* Far-field intensities |fft(exit)|^2 (unnormalised, no shift), float32
  * [probes][n], for exit waves complex64 [probes][n] (row stride ld). */

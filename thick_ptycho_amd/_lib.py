@@ -50,6 +50,17 @@ class TriScan(ctypes.Structure):
         ("out_zdir", c_int),
         ("ckpt_every", c_int),
         ("sigma", c_int),
+        ("rhs_steps", c_int),
+        ("gfwd", c_void_p),
+        ("grad", c_void_p),
+        ("gfwd_pstride", c_int64),
+        ("gfwd_zstride", c_int64),
+        ("grad_zstride", c_int64),
+        ("gfwd_z0", c_int),
+        ("gfwd_zdir", c_int),
+        ("grad_z0", c_int),
+        ("grad_zdir", c_int),
+        ("grad_skip_row0", c_int),
     ]
 
 
@@ -87,6 +98,7 @@ SIGNATURES = {
     "tp_ms_kernel": (c_int, [c_int, c_double, c_double, c_double, c_void_p, c_void_p]),
     "tp_ms_sweep": (c_int, [ctypes.POINTER(MsScan), c_int, c_void_p]),
     "tp_ms_step": (c_int, [ctypes.POINTER(MsScan), c_int, c_int, c_void_p]),
+    "tp_amplitude_seed": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "tp_farfield": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p]),
 }
 

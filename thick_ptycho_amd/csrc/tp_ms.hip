@@ -303,6 +303,44 @@ __global__ __launch_bounds__(64 * MS_PPB) void farfield_kernel(const float2* __r
   for (int m = 0; m < R; ++m) out[(size_t)p * N + lane + 64 * m] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
 }
 
+// Amplitude-misfit seed: U = fft(exit); loss = sum (|U| - a)^2;
+// seed = scale * unnormalised-ifft((|U| - a) U / |U|).
+template <int R>
+__global__ __launch_bounds__(64 * MS_PPB) void amplitude_seed_kernel(const float2* __restrict__ ex,
+                                                                     const float* __restrict__ amp, int probes,
+                                                                     float scale, float2* __restrict__ seed,
+                                                                     float* __restrict__ loss) {
+  constexpr int N = 64 * R;
+  __shared__ float2 tw[N];
+  __shared__ float2 buf[MS_PPB][2][N];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * MS_PPB + wave;
+  build_twiddles(tw, N);
+  __syncthreads();
+  if (p >= probes) return;
+  float2 x[R];
+#pragma unroll
+  for (int m = 0; m < R; ++m) x[m] = ex[(size_t)p * N + lane + 64 * m];
+  fft_wave<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+  float part = 0.f;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const float a = amp[(size_t)p * N + lane + 64 * m];
+    const float mag = sqrtf(fmaf(x[m].x, x[m].x, x[m].y * x[m].y));
+    const float dlt = mag - a;
+    part = fmaf(dlt, dlt, part);
+    const float w = mag > 0.f ? scale * dlt / mag : 0.f;
+    x[m] = cscale(x[m], w);
+  }
+  fft_wave<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+#pragma unroll
+  for (int m = 0; m < R; ++m) seed[(size_t)p * N + lane + 64 * m] = x[m];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if (lane == 0) loss[p] = part;
+}
+
 __global__ void ms_transmission_kernel(const double2* __restrict__ nidx, int n, int nz, double k, double dz,
                                        double nmed, float2* __restrict__ T) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -437,6 +475,33 @@ extern "C" int tp_ms_step(const tp_ms_scan* scans, int nscans, int z, void* stre
     case 8: hipLaunchKernelGGL((ms_step_kernel<8>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
     case 16: hipLaunchKernelGGL((ms_step_kernel<16>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
     case 32: hipLaunchKernelGGL((ms_step_kernel<32>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
+  }
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_amplitude_seed(const void* d_exit, const void* d_sqrt_i, int n, int probes, double scale,
+                                 void* d_seed, void* d_loss, void* stream) {
+  int r = 0;
+  int rc = ms_radix(n, r);
+  if (rc) return rc;
+  if (!d_exit || !d_sqrt_i || !d_seed || !d_loss || probes < 0) {
+    set_error("tp_amplitude_seed: invalid arguments");
+    return TP_ERR_INVALID;
+  }
+  if (probes == 0) return TP_OK;
+  const dim3 grid((probes + MS_PPB - 1) / MS_PPB), block(64 * MS_PPB);
+  hipStream_t st = (hipStream_t)stream;
+  const float2* ex = (const float2*)d_exit;
+  const float* a = (const float*)d_sqrt_i;
+  float2* sd = (float2*)d_seed;
+  float* ls = (float*)d_loss;
+  const float sc = (float)scale;
+  switch (r) {
+    case 4: hipLaunchKernelGGL((amplitude_seed_kernel<4>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
+    case 8: hipLaunchKernelGGL((amplitude_seed_kernel<8>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
+    case 16: hipLaunchKernelGGL((amplitude_seed_kernel<16>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
+    case 32: hipLaunchKernelGGL((amplitude_seed_kernel<32>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
   }
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;

@@ -416,7 +416,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // wavefronts; the next step's block is fetched into registers while the
 // current step computes, and the plane computed in step s is written at the
 // start of step s+1 (before that fetch) so the stores drain behind the math.
-template <int CH, int PPB, bool SIGMA>
+template <int CH, int PPB, bool SIGMA, bool GRAD>
 __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
@@ -440,6 +440,8 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   const int row0 = lane * CH;
   const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef);
   float2* tile = reinterpret_cast<float2*>(smem_v) + NA * NP + wave * TILE;
+  // gradient row accumulated over the workgroup's probes (tile_index layout)
+  float2* gbuf = reinterpret_cast<float2*>(smem_v) + NA * NP + PPB * TILE;
 
   f4v st[PER];
 #pragma unroll
@@ -471,6 +473,15 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   }
   __syncthreads();
 
+  float2 ga[CH], gb[CH];
+  if (GRAD) {
+    for (int idx = threadIdx.x; idx < TILE; idx += NT) gbuf[idx] = make_float2(0.f, 0.f);
+    const float2* g = (const float2*)d.gfwd + (size_t)(act ? p : 0) * d.gfwd_pstride + (int64_t)d.gfwd_z0 * d.gfwd_zstride;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) ga[m] = (act && row0 + m < n) ? g[row0 + m] : make_float2(0.f, 0.f);
+    __syncthreads();
+  }
+
   float2 rb[CH];
   for (int s = 1; s <= L; ++s) {
     // Plane s-1 is written first, then the next step's coefficients are
@@ -490,7 +501,13 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
         if (NV % NT == 0 || idx < NV) st[k] = src[idx];
       }
     }
-    if (d.rhs != nullptr) {
+    if (GRAD) {  // plane b = u_{s-1} of this step (prefetched before the math)
+      const float2* g =
+          (const float2*)d.gfwd + (size_t)(act ? p : 0) * d.gfwd_pstride + (int64_t)(d.gfwd_z0 + d.gfwd_zdir * s) * d.gfwd_zstride;
+#pragma unroll
+      for (int m = 0; m < CH; ++m) gb[m] = (act && row0 + m < n) ? g[row0 + m] : make_float2(0.f, 0.f);
+    }
+    if (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps)) {
       const float2* r = (const float2*)d.rhs + (size_t)(act ? p : 0) * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
       for (int m = 0; m < CH; ++m) rb[m] = (act && row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
@@ -498,7 +515,33 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
     } else {
       tri_step_lane<CH, false, SIGMA>(u, cf_l, rb, lane);
     }
+    int grow = 0;
+    if (GRAD) {
+      grow = d.grad_z0 + d.grad_zdir * (s - 1);
+      const bool skip_b = d.grad_skip_row0 && grow == 0;
+      if (act) {
+#pragma unroll
+        for (int m = 0; m < CH; ++m) {
+          const float2 w = skip_b ? ga[m] : cadd(ga[m], gb[m]);
+          const float2 c = cmul(cconj(u[m]), w);
+          float2* slot = gbuf + tile_index(row0 + m);
+          atomicAdd(&slot->x, c.x);
+          atomicAdd(&slot->y, c.y);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < CH; ++m) ga[m] = gb[m];
+    }
     __syncthreads();
+    if (GRAD) {  // one global atomic per row and component per workgroup
+      float* gr = (float*)((float2*)d.grad + (int64_t)grow * d.grad_zstride);
+      for (int i = threadIdx.x; i < n; i += NT) {
+        const float2 v = gbuf[tile_index(i)];
+        unsafeAtomicAdd(gr + 2 * i, v.x);
+        unsafeAtomicAdd(gr + 2 * i + 1, v.y);
+        gbuf[tile_index(i)] = make_float2(0.f, 0.f);
+      }
+    }
     if (s < L) {
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
@@ -557,7 +600,7 @@ __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s)
   }
   float2 rb[CH];
   const float2* cf = (const float2*)d.coef + (size_t)(s - 1) * A_COUNT * NP;
-  if (d.rhs != nullptr) {
+  if (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps)) {
     const float2* r = (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
     for (int m = 0; m < CH; ++m) rb[m] = (row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
@@ -624,6 +667,10 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
       set_error("scan %d: coef must be 16-byte aligned", i);
       return TP_ERR_INVALID;
     }
+    if ((d.grad != nullptr) != (scans[0].grad != nullptr) || ((d.grad != nullptr) && d.gfwd == nullptr)) {
+      set_error("scan %d: gradient accumulation needs grad and gfwd, on every scan of the launch", i);
+      return TP_ERR_INVALID;
+    }
     a.scans[i] = d;
     a.blk_start[i] = nblk;
     nblk += (d.probes + ppb - 1) / ppb;
@@ -635,31 +682,38 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
   return TP_OK;
 }
 
-template <int CH, int PPB, bool SIGMA>
+template <int CH, int PPB, bool SIGMA, bool GRAD>
 static int launch_sweep(const SweepArgs& a, hipStream_t st) {
   constexpr int NP = 64 * CH;
-  const size_t lds = ((size_t)(SIGMA ? 6 : 5) * NP + (size_t)PPB * (NP + NP / 8)) * sizeof(float2);
+  const size_t lds =
+      ((size_t)(SIGMA ? 6 : 5) * NP + (size_t)(PPB + (GRAD ? 1 : 0)) * (NP + NP / 8)) * sizeof(float2);
   if (lds > 163840) {
     set_error("chunk %d with %d probes per block needs %zu bytes of LDS (> 160 KiB); use fewer probes per block",
               CH, PPB, lds);
     return TP_ERR_INVALID;
   }
   if (lds > 65536) {
-    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA>,
+    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA, GRAD>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
+  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA, GRAD>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
 
 template <int CH, bool SIGMA>
-static int sweep_ppb(const SweepArgs& a, int ppb, hipStream_t st) {
+static int sweep_ppb(const SweepArgs& a, int ppb, bool grad, hipStream_t st) {
+  if (grad) {  // gradient accumulation is built for 1 and 4 probes per workgroup
+    if (ppb == 1) return launch_sweep<CH, 1, SIGMA, true>(a, st);
+    if (ppb == 4) return launch_sweep<CH, 4, SIGMA, true>(a, st);
+    set_error("gradient sweeps need probes_per_block 1 or 4 (got %d)", ppb);
+    return TP_ERR_INVALID;
+  }
   switch (ppb) {
-    case 1: return launch_sweep<CH, 1, SIGMA>(a, st);
-    case 2: return launch_sweep<CH, 2, SIGMA>(a, st);
-    case 4: return launch_sweep<CH, 4, SIGMA>(a, st);
-    case 8: return launch_sweep<CH, 8, SIGMA>(a, st);
+    case 1: return launch_sweep<CH, 1, SIGMA, false>(a, st);
+    case 2: return launch_sweep<CH, 2, SIGMA, false>(a, st);
+    case 4: return launch_sweep<CH, 4, SIGMA, false>(a, st);
+    case 8: return launch_sweep<CH, 8, SIGMA, false>(a, st);
     default:
       set_error("probes_per_block must be 1, 2, 4 or 8 (got %d)", ppb);
       return TP_ERR_INVALID;
@@ -667,8 +721,8 @@ static int sweep_ppb(const SweepArgs& a, int ppb, hipStream_t st) {
 }
 
 template <int CH>
-static int sweep_sigma(const SweepArgs& a, int ppb, int sigma, hipStream_t st) {
-  return sigma ? sweep_ppb<CH, true>(a, ppb, st) : sweep_ppb<CH, false>(a, ppb, st);
+static int sweep_sigma(const SweepArgs& a, int ppb, int sigma, bool grad, hipStream_t st) {
+  return sigma ? sweep_ppb<CH, true>(a, ppb, grad, st) : sweep_ppb<CH, false>(a, ppb, grad, st);
 }
 
 }  // namespace tp
@@ -731,13 +785,14 @@ extern "C" int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int
   if (rc) return rc;
   if (a.nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
+  const bool grad = scans[0].grad != nullptr;
   switch (chunk) {
-    case 1: return sweep_sigma<1>(a, probes_per_block, sigma, st);
-    case 2: return sweep_sigma<2>(a, probes_per_block, sigma, st);
-    case 4: return sweep_sigma<4>(a, probes_per_block, sigma, st);
-    case 8: return sweep_sigma<8>(a, probes_per_block, sigma, st);
-    case 16: return sweep_sigma<16>(a, probes_per_block, sigma, st);
-    case 32: return sweep_sigma<32>(a, probes_per_block, sigma, st);
+    case 1: return sweep_sigma<1>(a, probes_per_block, sigma, grad, st);
+    case 2: return sweep_sigma<2>(a, probes_per_block, sigma, grad, st);
+    case 4: return sweep_sigma<4>(a, probes_per_block, sigma, grad, st);
+    case 8: return sweep_sigma<8>(a, probes_per_block, sigma, grad, st);
+    case 16: return sweep_sigma<16>(a, probes_per_block, sigma, grad, st);
+    case 32: return sweep_sigma<32>(a, probes_per_block, sigma, grad, st);
   }
   return TP_ERR_INVALID;
 }
@@ -751,8 +806,8 @@ extern "C" int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int 
   rc = build_args(scans, nscans, chunk, ppb, a, sigma);
   if (rc) return rc;
   for (int i = 0; i < nscans; ++i) {
-    if (!scans[i].out || s < 1 || s > scans[i].steps) {
-      set_error("tp_tri_step: scan %d needs out != NULL and 1 <= s <= steps (s=%d)", i, s);
+    if (!scans[i].out || s < 1 || s > scans[i].steps || scans[i].grad) {
+      set_error("tp_tri_step: scan %d needs out != NULL, 1 <= s <= steps (s=%d) and no gradient", i, s);
       return TP_ERR_INVALID;
     }
   }

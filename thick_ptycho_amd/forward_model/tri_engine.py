@@ -84,6 +84,14 @@ class TriScanSpec:
     ckpt_every: int = 0
     rhs: Optional[torch.Tensor] = None
     flip: bool = False
+    rhs_steps: int = 0
+    gfwd: Optional[torch.Tensor] = None
+    gfwd_z0: int = 0
+    gfwd_zdir: int = 1
+    grad: Optional[torch.Tensor] = None
+    grad_z0: int = 0
+    grad_zdir: int = 1
+    grad_skip_row0: bool = False
 
     def struct(self) -> _lib.TriScan:
         f = self.fac
@@ -96,10 +104,30 @@ class TriScanSpec:
         s.u0 = _lib.ptr(self.u0)
         s.u0_pstride = self.u0.stride(0) if P > 1 else n
         if self.rhs is not None:
-            assert self.rhs.dtype == torch.complex64 and self.rhs.shape[-2:] == (L, n) and self.rhs.stride(-1) == 1
+            nsteps = self.rhs_steps or L
+            assert self.rhs.dtype == torch.complex64 and self.rhs.dim() == 3 and self.rhs.stride(-1) == 1
+            assert self.rhs.shape[1] >= nsteps and self.rhs.shape[2] == n and self.rhs.shape[0] in (1, P)
             s.rhs = _lib.ptr(self.rhs)
             s.rhs_pstride = self.rhs.stride(0) if self.rhs.shape[0] > 1 else 0
             s.rhs_sstride = self.rhs.stride(1)
+            s.rhs_steps = self.rhs_steps
+        if self.grad is not None:
+            g = self.gfwd
+            assert g is not None and g.dtype == torch.complex64 and g.shape[0] == P and g.shape[2] == n
+            assert g.stride(-1) == 1 and 0 <= self.gfwd_z0 < g.shape[1] and 0 <= self.gfwd_z0 + self.gfwd_zdir * L < g.shape[1]
+            r = self.grad
+            assert r.dtype == torch.complex64 and r.dim() == 2 and r.shape[1] == n and r.stride(-1) == 1
+            assert 0 <= self.grad_z0 < r.shape[0] and 0 <= self.grad_z0 + self.grad_zdir * (L - 1) < r.shape[0]
+            s.gfwd = _lib.ptr(g)
+            s.gfwd_pstride = g.stride(0)
+            s.gfwd_zstride = g.stride(1)
+            s.gfwd_z0 = self.gfwd_z0
+            s.gfwd_zdir = self.gfwd_zdir
+            s.grad = _lib.ptr(r)
+            s.grad_zstride = r.stride(0)
+            s.grad_z0 = self.grad_z0
+            s.grad_zdir = self.grad_zdir
+            s.grad_skip_row0 = int(self.grad_skip_row0)
         if self.out is not None:
             assert self.out.dtype == torch.complex64 and self.out.shape == (P, L + 1, n) and self.out.stride(-1) == 1
             s.out = _lib.ptr(self.out)
@@ -121,6 +149,14 @@ class TriScanSpec:
         s.steps = L
         s.probes = P
         return s
+
+
+def step_slice(fac: TriFactors, first: int, count: int) -> TriFactors:
+    """View of ``fac`` restricted to steps first+1 .. first+count (no copy)."""
+    assert 0 <= first and first + count <= fac.steps and count >= 1
+    per = 6 * 64 * fac.chunk
+    return TriFactors(fac.coef[first * per:(first + count) * per], fac.n, count, fac.chunk, fac.mode, fac.b_sign,
+                      fac.status, fac.sigma, keep=[fac])
 
 
 def _structs(specs: Sequence[TriScanSpec]):

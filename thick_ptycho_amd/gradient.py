@@ -1,0 +1,202 @@
+"""Gradients of the far-field amplitude misfit through CN slice stepping (GPU).
+
+Loss (per scan):  J(n) = sum_p sum_k (|fft(u_L^p)|_k - sqrt(I_k^p))^2, with u_L the
+exit plane of the forward sweep.
+
+``amplitude_gradient`` returns the exact gradient dJ/dRe(n) + i dJ/dIm(n): forward
+sweep keeping every ``checkpoint``-th plane, the loss seed dJ/d conj(u_L) from
+``tp_amplitude_seed``, then the exact adjoint sweep segment by segment in
+reverse, each segment's forward planes recomputed from its checkpoint, with
+the per-slice accumulation sum_p conj(lambda_s) (u_s + u_{s-1}) fused into the
+adjoint kernel.  Memory: (L/checkpoint + 1) planes per probe instead of L + 1.
+
+``reference_gradient`` reproduces ReconstructorPWE.compute_grad_least_squares
+(thick_ptycho/reconstruction/pwe_reconstructor.py:100-164): the reference
+adjoint (quirk Q3), the exit-wave error of base_reconstructor.py:87-207 (the
+seed scaled by 1/N), g = g0 (u_s + u_{s-1}) without the u_0 term, and the
+(grad_re, grad_im) split with the reference's signs.
+
+``ExitWaves`` is a torch.autograd.Function mapping n -> exit waves whose
+backward runs the same checkpointed exact adjoint.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .forward_model import operators, tri_engine
+
+
+def object_contribution(n: torch.Tensor, k: float, dz: float) -> torch.Tensor:
+    """C[:, j-1] = (dz/2) (coef[:, j-1] + coef[:, j]) / 2 with coef = (k/i)(n - 1), on the GPU.
+
+    Same formula as base_simulation_space.py:315-383 with the default coefficient."""
+    coef = (k / 1j) * (n - 1)
+    return ((dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2).contiguous()
+
+
+def amplitude_seed(exit_w: torch.Tensor, sqrt_i: torch.Tensor, scale: float = 1.0) -> Tuple[torch.Tensor, torch.Tensor]:
+    """(seed, loss per probe): seed = scale * dJ/d conj(exit)."""
+    P, n = exit_w.shape
+    seed = torch.empty_like(exit_w)
+    loss = torch.empty(P, dtype=torch.float32, device=exit_w.device)
+    _lib.check(
+        _lib.load().tp_amplitude_seed(_lib.ptr(exit_w), _lib.ptr(sqrt_i), n, P, float(scale), _lib.ptr(seed),
+                                      _lib.ptr(loss), _lib.stream_handle()),
+        "tp_amplitude_seed",
+    )
+    return seed, loss
+
+
+class _Sweeper:
+    """Forward factors + adjoint factors of one object, and the checkpointed sweeps."""
+
+    def __init__(self, C: torch.Tensor, A, B, adjoint_mode: str, checkpoint: int, ppb: int = 4):
+        self.n, self.L = C.shape
+        self.fwd = tri_engine.factor(C, A, B, "forward")
+        self.adj = tri_engine.factor(C, A, B, adjoint_mode)
+        self.K = max(1, min(checkpoint if checkpoint > 0 else self.L, self.L))
+        self.ppb = ppb
+
+    def forward(self, u0: torch.Tensor):
+        P = u0.shape[0]
+        dev = u0.device
+        ck = torch.empty((P, self.L // self.K + 1, self.n), dtype=torch.complex64, device=dev)
+        ex = torch.empty((P, self.n), dtype=torch.complex64, device=dev)
+        tri_engine.sweep([tri_engine.TriScanSpec(self.fwd, u0, ckpt=ck, ckpt_every=self.K, exit=ex)],
+                         probes_per_block=self.ppb)
+        return ck, ex
+
+    def adjoint(self, ck: torch.Tensor, seed: torch.Tensor, skip_row0: bool = False) -> torch.Tensor:
+        """Accumulate G[s-1] = sum_p conj(lambda_s) (u_s + u_{s-1}) over s = L..1 (complex64 (L, n))."""
+        P = seed.shape[0]
+        dev = seed.device
+        L, K, n = self.L, self.K, self.n
+        grad = torch.zeros((L, n), dtype=torch.complex64, device=dev)
+        lam = torch.zeros((P, n), dtype=torch.complex64, device=dev)
+        seg = torch.empty((P, K + 1, n), dtype=torch.complex64, device=dev)
+        seed3 = seed.reshape(P, 1, n)
+        nseg = math.ceil(L / K)
+        for g in range(nseg - 1, -1, -1):
+            s0 = g * K
+            s1 = min(s0 + K, L)
+            ln = s1 - s0
+            segv = seg[:, : ln + 1]
+            fw = tri_engine.step_slice(self.fwd, s0, ln)
+            tri_engine.sweep([tri_engine.TriScanSpec(fw, ck[:, g], out=segv)], probes_per_block=self.ppb)
+            aw = tri_engine.step_slice(self.adj, L - s1, ln)
+            first = s1 == L
+            spec = tri_engine.TriScanSpec(
+                aw, lam, exit=lam,
+                rhs=seed3 if first else None, rhs_steps=1 if first else 0,
+                gfwd=segv, gfwd_z0=ln, gfwd_zdir=-1,
+                grad=grad, grad_z0=s1 - 1, grad_zdir=-1, grad_skip_row0=skip_row0,
+            )
+            tri_engine.sweep([spec], probes_per_block=self.ppb if self.ppb in (1, 4) else 4)
+        return grad
+
+
+def _operators(nx: int, k: float, dx: float, dz: float, bc):
+    return operators.step_operators_1d(nx, k, dx, dz, bc)
+
+
+def _dev_tensor(x, dtype, device):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device=device, dtype=dtype)
+
+
+def amplitude_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="dirichlet", checkpoint: int = 32,
+                       device=None, seed=None):
+    """(loss, grad) of J(n) for one scan; grad complex128 (N, nz) on the GPU in the
+    dJ/dRe(n) + i dJ/dIm(n) convention.
+
+    ``seed`` (P, N), when given, replaces the loss seed dJ/d conj(u_L) that is
+    otherwise computed from the exit waves and ``sqrt_i`` (the loss is still
+    evaluated from them)."""
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    n_t = _dev_tensor(n, torch.complex128, device)
+    N, nz = n_t.shape
+    A, B = _operators(N, k, dx, dz, bc)
+    C = object_contribution(n_t, k, dz)
+    sw = _Sweeper(C, A, B, "exact_adjoint", checkpoint)
+    u0 = _dev_tensor(probes, torch.complex64, device).reshape(-1, N)
+    amp = _dev_tensor(sqrt_i, torch.float32, device).reshape(u0.shape[0], N)
+    ck, ex = sw.forward(u0)
+    sd, loss = amplitude_seed(ex, amp, 1.0)
+    if seed is not None:
+        sd = _dev_tensor(seed, torch.complex64, device).reshape(u0.shape[0], N)
+    G = sw.adjoint(ck, sd)
+    Gn = torch.zeros((N, nz), dtype=torch.complex128, device=device)
+    Gt = G.to(torch.complex128).transpose(0, 1)
+    Gn[:, :-1] += Gt
+    Gn[:, 1:] += Gt
+    Gn *= (dz / 4) * (k / 1j)
+    return float(loss.double().sum().item()), 2 * torch.conj(Gn)
+
+
+def reference_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="dirichlet", checkpoint: int = 32,
+                       device=None, exit_error=None):
+    """(grad_real, grad_imag), each float64 (L*N,) step-major, as
+    ReconstructorPWE.compute_grad_least_squares with phase retrieval.
+
+    ``exit_error`` (P, N), when given, is the exit-wave error e = u_L - P_a(u_L)
+    supplied by the caller (the reference's ``exit_wave_error`` argument,
+    pwe_reconstructor.py:100); otherwise it is computed from the exit waves
+    and ``sqrt_i`` (base_reconstructor.py:87-121, 171-207) and ``sqrt_i`` may be None
+    only when ``exit_error`` is given."""
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    n_t = _dev_tensor(n, torch.complex128, device)
+    N, nz = n_t.shape
+    A, B = _operators(N, k, dx, dz, bc)
+    C = object_contribution(n_t, k, dz)
+    sw = _Sweeper(C, A, B, "adjoint", checkpoint)
+    u0 = _dev_tensor(probes, torch.complex64, device).reshape(-1, N)
+    ck, ex = sw.forward(u0)
+    if exit_error is not None:
+        err = _dev_tensor(exit_error, torch.complex64, device).reshape(u0.shape[0], N)
+    else:
+        amp = _dev_tensor(sqrt_i, torch.float32, device).reshape(u0.shape[0], N)
+        err, _ = amplitude_seed(ex, amp, 1.0 / N)
+    G = sw.adjoint(ck, err, skip_row0=True).to(torch.complex128)
+    g0 = dz * k / 2j
+    X = (g0 * G).reshape(-1)
+    return (-X.real).cpu().numpy(), X.imag.cpu().numpy()
+
+
+class ExitWaves(torch.autograd.Function):
+    """n (complex128 (N, nz), CUDA) -> exit planes (complex64 (P, N)) of the CN forward sweep.
+
+    backward runs the checkpointed exact adjoint; the returned gradient follows
+    torch's convention for complex inputs (conjugate Wirtinger)."""
+
+    @staticmethod
+    def forward(ctx, n, probes, k: float, dx: float, dz: float, bc: str = "dirichlet", checkpoint: int = 32):
+        N, nz = n.shape
+        A, B = _operators(N, k, dx, dz, bc)
+        C = object_contribution(n.detach(), k, dz)
+        sw = _Sweeper(C, A, B, "exact_adjoint", checkpoint)
+        u0 = probes.detach().to(torch.complex64).reshape(-1, N).contiguous()
+        ck, ex = sw.forward(u0)
+        ctx.sw = sw
+        ctx.k, ctx.dz, ctx.nz = k, dz, nz
+        ctx.save_for_backward(ck)
+        return ex
+
+    @staticmethod
+    def backward(ctx, grad_exit):
+        (ck,) = ctx.saved_tensors
+        sw = ctx.sw
+        G = sw.adjoint(ck, grad_exit.to(torch.complex64).contiguous())
+        N = G.shape[1]
+        Gn = torch.zeros((N, ctx.nz), dtype=torch.complex128, device=G.device)
+        Gt = G.to(torch.complex128).transpose(0, 1)
+        Gn[:, :-1] += Gt
+        Gn[:, 1:] += Gt
+        Gn *= (ctx.dz / 4) * (ctx.k / 1j)
+        return torch.conj(Gn), None, None, None, None, None, None

@@ -156,17 +156,22 @@ def config1(nz: int = 1024, main_probes: int = 512, extra_probes: int = 128) -> 
 def config2(nz: int = 512, probes: int = 256) -> Workload:
     """2D reconstruction gradient: Nx = 1024, Nz = 512, 256 probes at 2 + 4 s.
 
-    ``extra`` holds the measured intensities simulated from the true object
-    with delta-n perturbed by 5 % Gaussian noise (seed 21), and an initial
-    guess (free space)."""
+    ``extra`` holds the true object, the object the measured intensities are
+    simulated from (delta-n perturbed by 5 % Gaussian noise, seed 21) and the
+    free-space object.  The scan's object, where the gradient is evaluated,
+    is a seeded initial guess: 8 blobs from seed 22 with the config-0
+    generator.  (At free space the amplitude-misfit gradient is dominated by
+    detector bins where |F u_L| is ~1e-6 of its peak while the data are not,
+    so its phase there is set by complex64 rounding of u_L.)"""
     nx = 1024
     true = scan_2d(nx, nz, probes, [2 + 4 * i for i in range(probes)], seed=20, name="true")
     rng = np.random.default_rng(21)
     dn = 1.0 - true.n
     n_meas = 1.0 - dn * (1.0 + 0.05 * rng.standard_normal(dn.shape))
-    guess = np.ones_like(true.n)
+    guess = blob_object_2d(nx, nz, 8, np.random.default_rng(22))
     s = Scan2D(nx=nx, nz=nz, n=guess, probes=true.probes, bc=true.bc, name="grad")
-    return Workload("2d-gradient-1024x512-256probes", [s], extra={"n_true": true.n, "n_meas_object": n_meas})
+    return Workload("2d-gradient-1024x512-256probes", [s],
+                    extra={"n_true": true.n, "n_meas_object": n_meas, "n_free": np.ones_like(true.n)})
 
 
 def raster_centres(n_side: int, step: float, centre: float) -> np.ndarray:
