@@ -37,6 +37,12 @@
  *                      with _propagate / _get_propagation_kernels (:49-78).
  *   tp_farfield     <- BaseForwardModelSolver.get_farfield_intensities
  *                      (forward_model/base/base_solver.py:213-249), noise off.
+ *   tp_band_factor  <- the same _construct_solve_cache / _build_lu_factors
+ *                      (iterative_lu.py:66-133) for two transverse axes, where
+ *                      A0 - D_j is the block-tridiagonal matrix of
+ *                      operator_matrices.py:75-126 (bandwidth nx).
+ *   tp_band_sweep   <- _solve_single_probe_impl (iterative_lu.py:138-197) for
+ *                      two transverse axes, all probes in one launch.
  */
 #ifndef THICK_PTYCHO_AMD_H
 #define THICK_PTYCHO_AMD_H
@@ -183,6 +189,66 @@ int tp_amplitude_seed(const void* d_exit, const void* d_sqrt_i, int n, int probe
 /* Far-field intensities |fft(exit)|^2 (unnormalised, no shift), float32
  * [probes][n], for exit waves complex64 [probes][n] (row stride ld). */
 int tp_farfield(const void* d_exit, int64_t ld, int n, int probes, void* d_intensity, void* stream);
+
+/* ------------------------------------------------------------------------
+ * CN stepping with two transverse axes (N = nx * ny unknowns per slice)
+ *
+ * The per-slice operator is block tridiagonal: ny diagonal blocks A_jj
+ * (tridiagonal nx x nx: x stencil plus the object term) coupled by scaled
+ * identities, A(j, j-1) = f_j I and A(j, j+1) = e_j I.  Flat index j*nx + i.
+ * tp_band_factor forms the block LU, S_0 = A_00, S_j = A_jj - f_j e_{j-1} S_{j-1}^{-1},
+ * in FP64 and keeps each Schur inverse W_j = S_j^{-1} on the band
+ * |col - row| <= w (exact band entries, from the banded LDU of S_j and the
+ * Takahashi recurrences); entries beyond the band are dropped, and
+ * d_edge[s] = max |W_j(i, i +- w)| / max |W_j(i, i)| over the slice's blocks
+ * measures what the band leaves out (the caller widens w until it is below
+ * single-precision rounding).  Band coefficients: complex64
+ * [steps][ny][2w+1][nx], element (d, i) = W_j(i, i + d - w).
+ * ------------------------------------------------------------------------ */
+
+int64_t tp_band_coef_bytes(int nx, int ny, int steps, int w);
+
+/* d_C: complex128 (N, ncol) row-major (create_object_contribution(...).reshape(-1, nz-1)).
+ * d_tri: complex128 [3][3][nx]: the x-stencil diagonals (sub, diag, sup) of the
+ *   first block, the interior blocks and the last block of the LHS base operator.
+ * d_cpl: complex128 [ny][2] = (f_j, e_j) of the LHS base operator.
+ * Block j of A(s) = tri[type(j)] + lhs_sign * diag(c_l(s)) on its diagonal, with
+ * c_l(s) = C[j*nx + i, lhs_col0 + col_step*(s-1)] (0 outside [0, ncol); conj_c
+ * conjugates).  with_sigma != 0 also stores d_sigma complex64 [steps][N],
+ * sigma = 2 + lhs_sign * c_l(s) + rhs_sign * c_r(s), c_r from rhs_col0 (the
+ * caller guarantees that the RHS base operator is 2I minus the LHS base).
+ * 1 <= w <= 32, w < nx, nx * (2w+1) * 16 bytes <= 160 KiB.  One wavefront per slice.
+ * d_edge: float32 [steps]; d_status: caller-zeroed int, set to 1 on a zero pivot. */
+int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const void* d_tri, const void* d_cpl,
+                   double lhs_sign, double rhs_sign, int conj_c, int lhs_col0, int rhs_col0, int col_step,
+                   int steps, int w, int with_sigma, void* d_W, void* d_sigma, float* d_edge, int* d_status,
+                   void* stream);
+
+/* One scan of a 3D sweep; strides in complex elements, planes hold N values. */
+typedef struct tp_band_scan {
+  const void* W;          /* tp_band_factor band coefficients                     */
+  const void* cpl;        /* complex64 [ny][2] = (f_j, e_j) of the factored LHS   */
+  const void* sigma;      /* tp_band_factor d_sigma, or NULL for sigma = 2         */
+  const void* u0;         /* initial planes, u0 + p*u0_pstride                    */
+  void* out;              /* full field or NULL: plane s at out + p*out_pstride +
+                             (out_z0 + out_zdir*s)*out_zstride, s = 0..steps     */
+  void* exitp;            /* last plane or NULL: exitp + p*exit_pstride          */
+  void* ckpt;             /* checkpoints or NULL (as tp_tri_scan)                */
+  const void* rhs;        /* extra RHS b_s or NULL: rhs + p*rhs_pstride + (s-1)*rhs_sstride */
+  void* work;             /* two planes per probe (work + p*work_pstride), required
+                             when out is NULL                                    */
+  int64_t u0_pstride, out_pstride, out_zstride, exit_pstride, ckpt_pstride, ckpt_zstride;
+  int64_t rhs_pstride, rhs_sstride, work_pstride;
+  int nx, ny, w, steps, probes;
+  int out_z0, out_zdir, ckpt_every;
+  int rhs_steps;          /* rhs applies to steps 1..rhs_steps (0 = every step) */
+  int reserved;
+} tp_band_scan;
+
+/* Persistent 3D sweep over every probe of every scan (all with the same w,
+ * w in {4, 6, 8, 12, 16, 24, 32}, nx <= 256, and the same sigma form): one workgroup per
+ * probe, u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
+int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream);
 
 #ifdef __cplusplus
 }

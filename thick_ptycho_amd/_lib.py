@@ -82,6 +82,39 @@ class MsScan(ctypes.Structure):
     ]
 
 
+class BandScan(ctypes.Structure):
+    _fields_ = [
+        ("W", c_void_p),
+        ("cpl", c_void_p),
+        ("sigma", c_void_p),
+        ("u0", c_void_p),
+        ("out", c_void_p),
+        ("exitp", c_void_p),
+        ("ckpt", c_void_p),
+        ("rhs", c_void_p),
+        ("work", c_void_p),
+        ("u0_pstride", c_int64),
+        ("out_pstride", c_int64),
+        ("out_zstride", c_int64),
+        ("exit_pstride", c_int64),
+        ("ckpt_pstride", c_int64),
+        ("ckpt_zstride", c_int64),
+        ("rhs_pstride", c_int64),
+        ("rhs_sstride", c_int64),
+        ("work_pstride", c_int64),
+        ("nx", c_int),
+        ("ny", c_int),
+        ("w", c_int),
+        ("steps", c_int),
+        ("probes", c_int),
+        ("out_z0", c_int),
+        ("out_zdir", c_int),
+        ("ckpt_every", c_int),
+        ("rhs_steps", c_int),
+        ("reserved", c_int),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "tp_abi_version": (c_int, []),
@@ -100,6 +133,13 @@ SIGNATURES = {
     "tp_ms_step": (c_int, [ctypes.POINTER(MsScan), c_int, c_int, c_void_p]),
     "tp_amplitude_seed": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "tp_farfield": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p]),
+    "tp_band_coef_bytes": (c_int64, [c_int, c_int, c_int, c_int]),
+    "tp_band_factor": (
+        c_int,
+        [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int,
+         c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_void_p]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

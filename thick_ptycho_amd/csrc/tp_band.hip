@@ -1,0 +1,482 @@
+// CN slice stepping with two transverse axes: the per-slice operator is block
+// tridiagonal (ny blocks of nx x nx),
+//
+//     A = [ A_00  e_0 I                      ]
+//         [ f_1 I A_11  e_1 I                ]     A_jj tridiagonal (x stencil + object term),
+//         [        ...    ...    ...         ]     couplings scaled identities (y stencil),
+//         [               f_{ny-1} I  A_{ny-1,ny-1} ]
+//
+// which is the reference's pentadiagonal-in-band matrix with bandwidth nx
+// (operator_matrices.py:75-126, boundary_conditions.py:96-130).
+//
+// Factor (once per slice, FP64): block LU, S_0 = A_00, S_j = A_jj - f_j e_{j-1} S_{j-1}^{-1}.
+// The Schur inverses W_j = S_j^{-1} of these diagonally dominant blocks decay
+// geometrically away from the diagonal; each W_j is kept on a band of
+// half-width w chosen by the caller.  The band of W_j is computed exactly from
+// the banded LDU factorisation of S_j (no fill inside the band) with the
+// Takahashi recurrences
+//     Z = D^{-1} L^{-1} + (I - U) Z,   Z = U^{-1} D^{-1} + Z (I - L),
+// which need only band entries of Z.  The largest |W_j| entry at distance w
+// (relative to the largest diagonal entry) is reported per slice so the caller
+// can widen the band until what is dropped lies below single-precision
+// rounding.  W_j is stored as complex64 diagonals: W[s][j][d][i] = W_j(i, i+d-w).
+//
+// Sweep (per probe, persistent over slices): with A + B = 2I (or diag(sigma)),
+//     u_s = 2 A^{-1} u_{s-1} - u_{s-1}      (sigma: A^{-1}(sigma u_{s-1}) - u_{s-1})
+// and A^{-1} r by block forward elimination / back substitution:
+//     y_j = r_j - f_j z_{j-1},  z_j = W_j y_j            (j = 0 .. ny-1)
+//     x_{ny-1} = z_{ny-1},      x_j = z_j - e_j W_j x_{j+1}
+// One workgroup per probe, one transverse row per thread; every global access
+// of a thread is to its own row of each block, so only the band mat-vec
+// exchanges data (through LDS).
+#include <algorithm>
+
+#include "tp_common.h"
+
+namespace tp {
+
+struct BandFactorArgs {
+  const double2* C;    // (N, ncol) row-major, N = nx * ny
+  const double2* tri;  // [3][3][nx]: block type (first, interior, last) x (sub, diag, sup)
+  const double2* cpl;  // [ny][2]: (f_j, e_j)
+  int nx, ny, ncol, steps, w;
+  double lsign, rsign;
+  int conj_c, lcol0, rcol0, cstep, with_sigma;
+  float2* W;
+  float2* sigma;
+  float* edge;
+  int* status;
+};
+
+__device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+__global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f) {
+  extern __shared__ double2 S[];  // band of the current block: S[i * BW + d] = M(i, i + d - w)
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int nx = f.nx, ny = f.ny, w = f.w, BW = 2 * w + 1;
+  const int lc = f.lcol0 + f.cstep * s;
+  const int rc = f.rcol0 + f.cstep * s;
+  const bool lok = lc >= 0 && lc < f.ncol;
+  const bool rok = rc >= 0 && rc < f.ncol;
+  const double2 zero = make_double2(0.0, 0.0);
+  double edge = 0.0, dmax = 0.0;
+  bool singular = false;
+
+  for (int j = 0; j < ny; ++j) {
+    const int type = (j == 0) ? 0 : (j == ny - 1 ? 2 : 1);
+    const double2* tri = f.tri + (size_t)type * 3 * nx;
+    const double2 fe = (j > 0) ? zmul(f.cpl[2 * j], f.cpl[2 * (j - 1) + 1]) : zero;
+    const double2 mfe = make_double2(-fe.x, -fe.y);
+    // S_j = A_jj - f_j e_{j-1} W_{j-1}  (W_{j-1} is still in S)
+    for (int t = lane; t < nx * BW; t += 64) {
+      const int i = t / BW, d = t - i * BW, col = i + d - w;
+      double2 v = zero;
+      if (col >= 0 && col < nx) {
+        if (j > 0) v = zmul(mfe, S[t]);
+        if (d == w - 1) {
+          v = zadd(v, tri[i]);
+        } else if (d == w) {
+          v = zadd(v, tri[nx + i]);
+          if (lok) {
+            double2 c = f.C[(size_t)(j * nx + i) * f.ncol + lc];
+            if (f.conj_c) c = zconj(c);
+            v = zadd(v, zscale(c, f.lsign));
+          }
+        } else if (d == w + 1) {
+          v = zadd(v, tri[2 * nx + i]);
+        }
+      }
+      S[t] = v;
+    }
+    __syncthreads();
+
+    // banded LDU, no pivoting (the blocks are strictly diagonally dominant)
+    for (int k = 0; k < nx; ++k) {
+      const int m = min(w, nx - 1 - k);
+      const double2 piv = S[k * BW + w];
+      if (piv.x == 0.0 && piv.y == 0.0) singular = true;
+      const double2 inv = zinv(piv);
+      for (int t = lane; t < m * m; t += 64) {
+        const int a = t / m, b = t - a * m;
+        const int i = k + 1 + a, c = k + 1 + b;
+        const double2 sik = S[i * BW + (k - i + w)];
+        const double2 skc = S[k * BW + (c - k + w)];
+        const int o = i * BW + (c - i + w);
+        S[o] = zsub(S[o], zmul(zmul(sik, skc), inv));
+      }
+      __syncthreads();
+      if (lane < m) {
+        const int o = (k + 1 + lane) * BW + (w - 1 - lane);  // L(k+1+lane, k)
+        S[o] = zmul(S[o], inv);
+      } else if (lane < 2 * m) {
+        const int o = k * BW + (w + 1 + lane - m);  // U(k, k+1+lane-m)
+        S[o] = zmul(S[o], inv);
+      }
+      __syncthreads();
+    }
+
+    // Takahashi: band of Z = S^{-1}, rows from the bottom
+    for (int i = nx - 1; i >= 0; --i) {
+      const int m = min(w, nx - 1 - i);
+      double2 val = zero, prod = zero;
+      if (lane < m) {  // Z(i, c) = -sum_r U(i, r) Z(r, c)
+        const int c = i + 1 + lane;
+        double2 acc = zero;
+        for (int q = 0; q < m; ++q) {
+          const int r = i + 1 + q;
+          acc = zadd(acc, zmul(S[i * BW + (r - i + w)], S[r * BW + (c - r + w)]));
+        }
+        val = make_double2(-acc.x, -acc.y);
+      } else if (lane < 2 * m) {  // Z(c, i) = -sum_r Z(c, r) L(r, i)
+        const int c = i + 1 + lane - m;
+        double2 acc = zero;
+        for (int q = 0; q < m; ++q) {
+          const int r = i + 1 + q;
+          acc = zadd(acc, zmul(S[c * BW + (r - c + w)], S[r * BW + (i - r + w)]));
+        }
+        val = make_double2(-acc.x, -acc.y);
+        prod = zmul(S[i * BW + (c - i + w)], val);  // U(i, c) Z(c, i)
+      }
+      double px = prod.x, py = prod.y;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        px += __shfl_xor(px, off, 64);
+        py += __shfl_xor(py, off, 64);
+      }
+      const double2 Di = S[i * BW + w];
+      __syncthreads();
+      if (lane < m) {
+        const int c = i + 1 + lane;
+        S[i * BW + (c - i + w)] = val;
+        if (lane == w - 1) edge = fmax(edge, zabs(val));
+      } else if (lane < 2 * m) {
+        const int c = i + 1 + lane - m;
+        S[c * BW + (i - c + w)] = val;
+        if (lane - m == w - 1) edge = fmax(edge, zabs(val));
+      }
+      if (lane == 0) {
+        const double2 z = zsub(zinv(Di), make_double2(px, py));
+        S[i * BW + w] = z;
+        dmax = fmax(dmax, zabs(z));
+      }
+      __syncthreads();
+    }
+
+    float2* Wout = f.W + ((size_t)s * ny + j) * BW * nx;
+    for (int t = lane; t < nx * BW; t += 64) {
+      const int d = t / nx, i = t - d * nx;
+      Wout[t] = to_f2(S[i * BW + d]);
+    }
+    __syncthreads();
+  }
+
+  if (f.with_sigma) {  // sigma = lhs diagonal + rhs diagonal = 2 + object terms of both slices
+    const int N = nx * ny;
+    float2* sg = f.sigma + (size_t)s * N;
+    for (int t = lane; t < N; t += 64) {
+      double2 v = make_double2(2.0, 0.0);
+      if (lok) {
+        double2 c = f.C[(size_t)t * f.ncol + lc];
+        if (f.conj_c) c = zconj(c);
+        v = zadd(v, zscale(c, f.lsign));
+      }
+      if (rok) {
+        double2 c = f.C[(size_t)t * f.ncol + rc];
+        if (f.conj_c) c = zconj(c);
+        v = zadd(v, zscale(c, f.rsign));
+      }
+      sg[t] = to_f2(v);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    edge = fmax(edge, __shfl_xor(edge, off, 64));
+    dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+  }
+  if (lane == 0) {
+    f.edge[s] = (float)(dmax > 0.0 ? edge / dmax : 0.0);
+    if (singular) atomicOr(f.status, 1);
+  }
+}
+
+// ------------------------------------------------------------------ sweep
+struct BandSweepArgs {
+  tp_band_scan scans[TP_MAX_SCANS];
+  int blk_start[TP_MAX_SCANS + 1];
+  int nscans, nblk;
+};
+
+__device__ __forceinline__ int band_find_scan(const BandSweepArgs& a, int blk) {
+  int sc = 0;
+#pragma unroll
+  for (int i = 1; i < TP_MAX_SCANS; ++i)
+    if (i < a.nscans && blk >= a.blk_start[i]) sc = i;
+  return sc;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; consecutive logical
+// blocks (probes of one scan, which read the same band coefficients) are kept on
+// one XCD's L2.
+__device__ __forceinline__ int band_xcd_block(int hw, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = hw & 7, j = hw >> 3;
+  return x * q + min(x, r) + j;
+}
+
+__device__ __forceinline__ float2* band_plane(const tp_band_scan& d, int p, int s, int64_t N) {
+  if (d.out) return (float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride;
+  return (float2*)d.work + (size_t)p * d.work_pstride + (int64_t)(s & 1) * N;
+}
+
+template <int W>
+__device__ __forceinline__ void load_band(float2 (&wv)[2 * W + 1], const float2* blk, int nx, int i, bool act) {
+#pragma unroll
+  for (int d = 0; d < 2 * W + 1; ++d) wv[d] = act ? blk[(size_t)d * nx + i] : make_float2(0.f, 0.f);
+}
+
+// z = sum_d W(i, i+d-w) v(i+d-w) with v staged in xb (offset W, zero pads).
+template <int W>
+__device__ __forceinline__ float2 band_mv(const float2 (&wv)[2 * W + 1], const float2* xb, int i) {
+  float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int d = 0; d < 2 * W + 1; ++d) z = cfma(wv[d], xb[i + d], z);
+  return z;
+}
+
+template <int W, bool SIGMA>
+__global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) {
+  constexpr int BW = 2 * W + 1;
+  extern __shared__ float2 xbuf[];  // two exchange buffers of NT + 2W
+  const int NT = blockDim.x;
+  const int blk = band_xcd_block(blockIdx.x, a.nblk);
+  const int sc = band_find_scan(a, blk);
+  const tp_band_scan& d = a.scans[sc];
+  const int p = blk - a.blk_start[sc];
+  const int i = threadIdx.x;
+  const int nx = d.nx, ny = d.ny, L = d.steps;
+  const int64_t N = (int64_t)nx * ny;
+  const bool act = i < nx;
+  const int XS = NT + 2 * W;
+  for (int t = i; t < 2 * XS; t += NT) xbuf[t] = make_float2(0.f, 0.f);
+  __syncthreads();
+  int xsel = 0;
+
+  const float2* cpl = (const float2*)d.cpl;
+  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
+  const float2 zero = make_float2(0.f, 0.f);
+
+  for (int s = 1; s <= L; ++s) {
+    const float2* prev = (s == 1) ? u0 : band_plane(d, p, s - 1, N);
+    float2* cur = band_plane(d, p, s, N);
+    const float2* Ws = (const float2*)d.W + (size_t)(s - 1) * ny * BW * nx;
+    const float2* sg = SIGMA ? (const float2*)d.sigma + (size_t)(s - 1) * N : nullptr;
+    const float2* rb = (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps))
+                           ? (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride
+                           : nullptr;
+    float2* out0 = (s == 1 && d.out) ? band_plane(d, p, 0, N) : nullptr;
+    float2* ck0 = (s == 1 && d.ckpt) ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride : nullptr;
+
+    // forward elimination
+    float2 wn[BW], wc[BW];
+    load_band<W>(wn, Ws, nx, i, act);
+    float2 un = act ? prev[i] : zero;
+    float2 sn = (SIGMA && act) ? sg[i] : zero;
+    float2 bn = (rb && act) ? rb[i] : zero;
+    float2 zprev = zero;
+    for (int j = 0; j < ny; ++j) {
+      const int64_t o = (int64_t)j * nx + i;
+#pragma unroll
+      for (int q = 0; q < BW; ++q) wc[q] = wn[q];
+      const float2 uc = un, sc_ = sn, bc = bn;
+      if (j + 1 < ny) {
+        load_band<W>(wn, Ws + (size_t)(j + 1) * BW * nx, nx, i, act);
+        un = act ? prev[o + nx] : zero;
+        if (SIGMA) sn = act ? sg[o + nx] : zero;
+        if (rb) bn = act ? rb[o + nx] : zero;
+      }
+      float2 y;
+      if (SIGMA) {
+        y = cmul(sc_, uc);
+        if (rb) y = cadd(y, bc);
+      } else {
+        y = rb ? cfma(make_float2(0.5f, 0.f), bc, uc) : uc;
+      }
+      if (j > 0) y = cfms(cpl[2 * j], zprev, y);
+      float2* xb = xbuf + xsel * XS;
+      xsel ^= 1;
+      xb[W + i] = act ? y : zero;
+      __syncthreads();
+      const float2 z = band_mv<W>(wc, xb, i);
+      if (act) {
+        cur[o] = z;
+        if (out0) out0[o] = uc;
+        if (ck0) ck0[o] = uc;
+      }
+      zprev = z;
+    }
+
+    // back substitution; the state update is fused into the store
+    float2* ex = (d.exitp && s == L) ? (float2*)d.exitp + (size_t)p * d.exit_pstride : nullptr;
+    float2* ck = (d.ckpt && (s % d.ckpt_every) == 0)
+                     ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride
+                     : nullptr;
+    load_band<W>(wn, Ws + (size_t)(ny - 1) * BW * nx, nx, i, act);
+    float2 zn = act ? cur[(int64_t)(ny - 1) * nx + i] : zero;
+    un = act ? prev[(int64_t)(ny - 1) * nx + i] : zero;
+    float2 xnext = zero;
+    for (int j = ny - 1; j >= 0; --j) {
+      const int64_t o = (int64_t)j * nx + i;
+#pragma unroll
+      for (int q = 0; q < BW; ++q) wc[q] = wn[q];
+      const float2 zc = zn, uc = un;
+      if (j > 0) {
+        load_band<W>(wn, Ws + (size_t)(j - 1) * BW * nx, nx, i, act);
+        zn = act ? cur[o - nx] : zero;
+        un = act ? prev[o - nx] : zero;
+      }
+      float2 x = zc;
+      if (j < ny - 1) {
+        float2* xb = xbuf + xsel * XS;
+        xsel ^= 1;
+        xb[W + i] = act ? xnext : zero;
+        __syncthreads();
+        x = cfms(cpl[2 * j + 1], band_mv<W>(wc, xb, i), zc);
+      }
+      const float2 un_ = SIGMA ? csub(x, uc) : csub(cscale(x, 2.f), uc);
+      if (act) {
+        cur[o] = un_;
+        if (ex) ex[o] = un_;
+        if (ck) ck[o] = un_;
+      }
+      xnext = x;
+    }
+  }
+}
+
+template <int W, bool SIGMA>
+static int launch_band_sweep(const BandSweepArgs& a, int nt, hipStream_t st) {
+  const size_t lds = (size_t)2 * (nt + 2 * W) * sizeof(float2);
+  hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA>), dim3(a.nblk), dim3(nt), lds, st, a);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+static bool band_w_supported(int w) {
+  return w == 4 || w == 6 || w == 8 || w == 12 || w == 16 || w == 24 || w == 32;
+}
+
+}  // namespace tp
+
+using namespace tp;
+
+extern "C" int64_t tp_band_coef_bytes(int nx, int ny, int steps, int w) {
+  return (int64_t)steps * ny * (2 * w + 1) * nx * (int64_t)sizeof(float2);
+}
+
+extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const void* d_tri, const void* d_cpl,
+                              double lhs_sign, double rhs_sign, int conj_c, int lhs_col0, int rhs_col0, int col_step,
+                              int steps, int w, int with_sigma, void* d_W, void* d_sigma, float* d_edge,
+                              int* d_status, void* stream) {
+  if (!d_C || !d_tri || !d_cpl || !d_W || !d_edge || !d_status || (with_sigma && !d_sigma)) {
+    set_error("tp_band_factor: null pointer");
+    return TP_ERR_INVALID;
+  }
+  if (nx < 2 || ny < 2 || steps < 1 || ncol < 1 || w < 1 || w > 32 || w >= nx) {
+    set_error("tp_band_factor: invalid sizes (nx=%d, ny=%d, steps=%d, ncol=%d, w=%d; need nx, ny >= 2, 1 <= w <= 32, "
+              "w < nx)", nx, ny, steps, ncol, w);
+    return TP_ERR_INVALID;
+  }
+  const size_t lds = (size_t)nx * (2 * w + 1) * sizeof(double2);
+  if (lds > 163840) {
+    set_error("tp_band_factor: band of %d rows x %d diagonals needs %zu bytes of LDS (> 160 KiB)", nx, 2 * w + 1, lds);
+    return TP_ERR_UNSUPPORTED;
+  }
+  BandFactorArgs f;
+  f.C = (const double2*)d_C;
+  f.tri = (const double2*)d_tri;
+  f.cpl = (const double2*)d_cpl;
+  f.nx = nx;
+  f.ny = ny;
+  f.ncol = ncol;
+  f.steps = steps;
+  f.w = w;
+  f.lsign = lhs_sign;
+  f.rsign = rhs_sign;
+  f.conj_c = conj_c;
+  f.lcol0 = lhs_col0;
+  f.rcol0 = rhs_col0;
+  f.cstep = col_step;
+  f.with_sigma = with_sigma;
+  f.W = (float2*)d_W;
+  f.sigma = (float2*)d_sigma;
+  f.edge = d_edge;
+  f.status = d_status;
+  if (lds > 65536) {
+    TP_CHECK_HIP(hipFuncSetAttribute((const void*)band_factor_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+  }
+  hipLaunchKernelGGL(band_factor_kernel, dim3(steps), dim3(64), lds, (hipStream_t)stream, f);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream) {
+  if (!scans || nscans < 1 || nscans > TP_MAX_SCANS) {
+    set_error("tp_band_sweep: nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
+    return TP_ERR_INVALID;
+  }
+  BandSweepArgs a;
+  memset(&a, 0, sizeof(a));
+  const int w = scans[0].w;
+  const bool sigma = scans[0].sigma != nullptr;
+  int nt = 64, nblk = 0;
+  for (int k = 0; k < nscans; ++k) {
+    const tp_band_scan& d = scans[k];
+    if (d.w != w || (d.sigma != nullptr) != sigma) {
+      set_error("tp_band_sweep: scan %d: all scans of one launch must share the band width and the sigma form", k);
+      return TP_ERR_INVALID;
+    }
+    if (!band_w_supported(d.w)) {
+      set_error("tp_band_sweep: band half-width %d not built (4, 6, 8, 12, 16, 24, 32)", d.w);
+      return TP_ERR_UNSUPPORTED;
+    }
+    if (d.nx < 2 || d.nx > 256 || d.ny < 2 || d.steps < 1 || d.probes < 0 || !d.W || !d.cpl || !d.u0) {
+      set_error("tp_band_sweep: scan %d: need 2 <= nx <= 256, ny >= 2, steps >= 1, W, cpl and u0 (nx=%d, ny=%d)", k,
+                d.nx, d.ny);
+      return TP_ERR_INVALID;
+    }
+    if (!d.out && !d.work) {
+      set_error("tp_band_sweep: scan %d: without full-field output a work buffer of two planes per probe is required",
+                k);
+      return TP_ERR_INVALID;
+    }
+    if (d.ckpt && d.ckpt_every < 1) {
+      set_error("tp_band_sweep: scan %d: ckpt_every must be >= 1", k);
+      return TP_ERR_INVALID;
+    }
+    nt = std::max(nt, (d.nx + 63) / 64 * 64);
+    a.scans[k] = d;
+    a.blk_start[k] = nblk;
+    nblk += d.probes;
+  }
+  a.blk_start[nscans] = nblk;
+  a.nscans = nscans;
+  a.nblk = nblk;
+  if (nblk == 0) return TP_OK;
+  hipStream_t st = (hipStream_t)stream;
+#define TP_BAND_CASE(WV)                                                                           \
+  case WV:                                                                                        \
+    return sigma ? launch_band_sweep<WV, true>(a, nt, st) : launch_band_sweep<WV, false>(a, nt, st);
+  switch (w) {
+    TP_BAND_CASE(4)
+    TP_BAND_CASE(6)
+    TP_BAND_CASE(8)
+    TP_BAND_CASE(12)
+    TP_BAND_CASE(16)
+    TP_BAND_CASE(24)
+    TP_BAND_CASE(32)
+  }
+#undef TP_BAND_CASE
+  return TP_ERR_UNSUPPORTED;
+}

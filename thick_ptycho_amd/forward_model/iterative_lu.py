@@ -1,9 +1,10 @@
 """Drop-in ``PWEIterativeLUSolver`` running on the MI355X kernels.
 
 Mirrors thick_ptycho/forward_model/pwe/solvers/iterative_lu.py:15-197:
-  _construct_solve_cache  -> tp_tri_factor (all slices, FP64 pivots, one launch)
-  _solve_single_probe_impl -> tp_tri_sweep with one probe
-  solve                    -> tp_tri_sweep with every probe/angle/projection in one launch
+  _construct_solve_cache  -> tp_tri_factor (one transverse axis) or tp_band_factor
+                             (two axes); all slices, FP64 factorisation, one launch
+  _solve_single_probe_impl -> tp_tri_sweep / tp_band_sweep with one probe
+  solve                    -> one persistent sweep over every probe/angle/projection
 """
 
 from __future__ import annotations
@@ -16,7 +17,7 @@ import numpy as np
 import scipy.sparse as sp
 import torch
 
-from . import tri_engine
+from . import band_engine, tri_engine
 from .base_solver import BasePWESolver
 from .operators import BoundaryType, Tridiag, chunk_for, step_operators_1d
 
@@ -29,14 +30,20 @@ class PWEForwardModel:
         self.simulation_space = simulation_space
         self.bc_type = bc_type
         self.block_size = simulation_space.block_size
-        if simulation_space.dimension != 2:
-            raise NotImplementedError("two transverse axes (banded per-slice operator) are not built in this path")
-        n = simulation_space.effective_shape[0]
-        self.A, self.B = step_operators_1d(n, simulation_space.k, simulation_space.dx, simulation_space.dz, bc_type)
+        ss = simulation_space
+        if ss.dimension == 2:
+            self.A, self.B = step_operators_1d(ss.effective_shape[0], ss.k, ss.dx, ss.dz, bc_type)
+        elif ss.dimension == 3:
+            nx, ny = ss.effective_shape[0], ss.effective_shape[1]
+            self.A, self.B = band_engine.step_operators_3d(nx, ny, ss.k, ss.dx, ss.dy, ss.dz, bc_type)
+        else:
+            raise ValueError("Unsupported dimension")
         self._cache = None
 
     @staticmethod
-    def _to_sparse(t: Tridiag) -> sp.csr_matrix:
+    def _to_sparse(t) -> sp.csr_matrix:
+        if isinstance(t, band_engine.Block3D):
+            return t.to_sparse()
         n = t.diag.shape[0]
         return sp.diags([t.sub[1:], t.diag, t.sup[:-1]], offsets=[-1, 0, 1], shape=(n, n), format="csr")
 
@@ -58,7 +65,7 @@ class PWEIterativeLUSolverCache:
     """Per (projection, mode) cache: the device factors of every slice."""
 
     cached_n: Optional[np.ndarray] = None
-    factors: Optional[tri_engine.TriFactors] = None
+    factors: Optional[object] = None  # tri_engine.TriFactors or band_engine.BandFactors
     b: Optional[np.ndarray] = None
 
     def reset(self, n=None):
@@ -78,7 +85,8 @@ class PWEIterativeLUSolver(BasePWESolver):
                          use_logging=use_logging, verbose=verbose, log=log, test_bcs=test_bcs, device=device)
         self.pwe_finite_differences = PWEForwardModel(simulation_space, bc_type=bc_type)
         self.probes_per_block = probes_per_block
-        self.chunk = chunk_for(self.block_size)
+        self.dimension = simulation_space.dimension
+        self.chunk = chunk_for(self.block_size) if self.dimension == 2 else None
 
     # ------------------------------------------------------------------ factors
     def object_contribution_device(self, n, scan_idx: int = 0) -> torch.Tensor:
@@ -89,7 +97,10 @@ class PWEIterativeLUSolver(BasePWESolver):
         t0 = time.time()
         fm = self.pwe_finite_differences
         C = self.object_contribution_device(n, scan_idx)
-        fac = tri_engine.factor(C, fm.A, fm.B, mode, chunk=self.chunk)
+        if self.dimension == 3:
+            fac = band_engine.factor(C, fm.A, fm.B, mode)
+        else:
+            fac = tri_engine.factor(C, fm.A, fm.B, mode, chunk=self.chunk)
         cache = self.projection_cache[proj_idx].modes[mode]
         cache.factors = fac
         cache.b = np.zeros(self.block_size, dtype=complex) * fac.b_sign
@@ -137,10 +148,23 @@ class PWEIterativeLUSolver(BasePWESolver):
             probe = np.zeros((N,), dtype=complex)
         u0 = self._probe_tensor(np.asarray(probe).reshape(1, N))
         out = torch.empty((1, self.nz, N), dtype=torch.complex64, device=self.device)
-        spec = tri_engine.TriScanSpec(fac, u0, out=out, rhs=self._rhs_steps(mode, rhs_block, False),
-                                      flip=(mode == "adjoint"))
-        tri_engine.sweep([spec], probes_per_block=1)
+        spec = self._spec(fac, u0, out=out, rhs=self._rhs_steps(mode, rhs_block, False), flip=(mode == "adjoint"))
+        self._launch([spec], probes_per_block=1)
         return out[0].transpose(0, 1).cpu().numpy().astype(np.complex128)
+
+    # ------------------------------------------------------------------ engine dispatch
+    def _spec(self, fac, u0, **kw):
+        if self.dimension == 3:
+            return band_engine.BandScanSpec(fac, u0, **kw)
+        return tri_engine.TriScanSpec(fac, u0, **kw)
+
+    def _launch(self, specs, probes_per_block: int = 4, plain: bool = False) -> None:
+        if self.dimension == 3:
+            band_engine.sweep(specs)
+        elif plain:
+            tri_engine.sweep_plain(specs)
+        else:
+            tri_engine.sweep(specs, probes_per_block=probes_per_block)
 
     # ------------------------------------------------------------------ batched
     def solve_device(self, n=None, mode: str = "forward", rhs_block=None, probes=None, keep: str = "all",
@@ -169,16 +193,12 @@ class PWEIterativeLUSolver(BasePWESolver):
             for a in range(A):
                 u0 = pr[a]
                 if keep == "all":
-                    specs.append(tri_engine.TriScanSpec(fac, u0, out=out[p, a], rhs=rhs, flip=(mode == "adjoint")))
+                    specs.append(self._spec(fac, u0, out=out[p, a], rhs=rhs, flip=(mode == "adjoint")))
                 else:
-                    specs.append(tri_engine.TriScanSpec(fac, u0, exit=out[p, a, :, 0], rhs=rhs))
+                    specs.append(self._spec(fac, u0, exit=out[p, a, :, 0], rhs=rhs))
         t0 = time.time()
         for i in range(0, len(specs), 8):
-            chunk_specs = specs[i:i + 8]
-            if plain:
-                tri_engine.sweep_plain(chunk_specs)
-            else:
-                tri_engine.sweep(chunk_specs, probes_per_block=self.probes_per_block)
+            self._launch(specs[i:i + 8], probes_per_block=self.probes_per_block, plain=plain)
         if self.verbose:
             torch.cuda.synchronize()
             self._timed(f"stepped {len(specs) * S} probes x {self.nz - 1} slices ({mode})", t0)
