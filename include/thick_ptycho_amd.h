@@ -203,7 +203,8 @@ int tp_farfield(const void* d_exit, int64_t ld, int n, int probes, void* d_inten
  * d_edge[s] = max |W_j(i, i +- w)| / max |W_j(i, i)| over the slice's blocks
  * measures what the band leaves out (the caller widens w until it is below
  * single-precision rounding).  Band coefficients: complex64
- * [steps][ny][2w+1][nx], element (d, i) = W_j(i, i + d - w).
+ * [steps][ny][w+1][nx][2]: diagonal d of W_j at pair d/2, slot d%2, i.e.
+ * W_j(i, i + d - w) at ((d/2)*nx + i)*2 + d%2 (d = 2w+1 is a zero pad).
  * ------------------------------------------------------------------------ */
 
 int64_t tp_band_coef_bytes(int nx, int ny, int steps, int w);

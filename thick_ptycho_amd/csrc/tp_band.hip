@@ -19,7 +19,8 @@
 // which need only band entries of Z.  The largest |W_j| entry at distance w
 // (relative to the largest diagonal entry) is reported per slice so the caller
 // can widen the band until what is dropped lies below single-precision
-// rounding.  W_j is stored as complex64 diagonals: W[s][j][d][i] = W_j(i, i+d-w).
+// rounding.  W_j is stored as complex64 diagonals in pairs (16-byte loads):
+// W_j(i, i+d-w) at W[s][j] + ((d/2) * nx + i) * 2 + d % 2, d = 0 .. 2w (+1 zero).
 //
 // Sweep (per probe, persistent over slices): with A + B = 2I (or diag(sigma)),
 //     u_s = 2 A^{-1} u_{s-1} - u_{s-1}      (sigma: A^{-1}(sigma u_{s-1}) - u_{s-1})
@@ -163,10 +164,11 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
       __syncthreads();
     }
 
-    float2* Wout = f.W + ((size_t)s * ny + j) * BW * nx;
-    for (int t = lane; t < nx * BW; t += 64) {
-      const int d = t / nx, i = t - d * nx;
-      Wout[t] = to_f2(S[i * BW + d]);
+    // pair layout: (d, i) at ((d/2) * nx + i) * 2 + d % 2, one padding diagonal
+    float2* Wout = f.W + ((size_t)s * ny + j) * (BW + 1) * nx;
+    for (int t = lane; t < nx * (BW + 1); t += 64) {
+      const int pr = t >> 1, dp = pr / nx, i = pr - dp * nx, d = 2 * dp + (t & 1);
+      Wout[t] = d < BW ? to_f2(S[i * BW + d]) : make_float2(0.f, 0.f);
     }
     __syncthreads();
   }
@@ -229,29 +231,165 @@ __device__ __forceinline__ float2* band_plane(const tp_band_scan& d, int p, int 
   return (float2*)d.work + (size_t)p * d.work_pstride + (int64_t)(s & 1) * N;
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// One thread's row of one block: the 2W+1 band coefficients as W+1 pairs
+// (16-byte loads; the pair layout puts (d, i) at ((d/2) * nx + i) * 2 + d % 2).
 template <int W>
-__device__ __forceinline__ void load_band(float2 (&wv)[2 * W + 1], const float2* blk, int nx, int i, bool act) {
+struct BandRow {
+  f4v p[W + 1];
+};
+
+template <int W>
+__device__ __forceinline__ void load_row(BandRow<W>& r, const float2* blk, int nx, int i, bool ok) {
+  const f4v* b4 = reinterpret_cast<const f4v*>(blk);
 #pragma unroll
-  for (int d = 0; d < 2 * W + 1; ++d) wv[d] = act ? blk[(size_t)d * nx + i] : make_float2(0.f, 0.f);
+  for (int q = 0; q <= W; ++q) r.p[q] = ok ? b4[(size_t)q * nx + i] : f4v{0.f, 0.f, 0.f, 0.f};
 }
 
-// z = sum_d W(i, i+d-w) v(i+d-w) with v staged in xb (offset W, zero pads).
+// z = sum_d W(i, i+d-W) v(i+d-W), v staged in xb at offset W with zero pads.
 template <int W>
-__device__ __forceinline__ float2 band_mv(const float2 (&wv)[2 * W + 1], const float2* xb, int i) {
+__device__ __forceinline__ float2 row_mv(const BandRow<W>& r, const float2* xb, int i) {
   float2 z = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int d = 0; d < 2 * W + 1; ++d) z = cfma(wv[d], xb[i + d], z);
+  for (int q = 0; q <= W; ++q) {
+    z = cfma(make_float2(r.p[q].x, r.p[q].y), xb[i + 2 * q], z);
+    if (2 * q + 1 < 2 * W + 1) z = cfma(make_float2(r.p[q].z, r.p[q].w), xb[i + 2 * q + 1], z);
+  }
   return z;
 }
 
-template <int W, bool SIGMA>
+// Blocks are processed in groups of G; the next group's operands are fetched
+// into a second register bank while the current group computes, so a step
+// waits only on loads issued a whole group earlier.  G keeps one group's loads
+// below the 63 outstanding vector-memory operations the wait counter tracks.
+template <int W>
+struct BandGroup {
+  // two banks of G rows of W+1 16-byte pairs (+ 4 scalars) stay within 256 VGPRs
+  static constexpr int G = W <= 4 ? 4 : (W <= 6 ? 3 : (W <= 12 ? 2 : 1));
+};
+
+template <int W, int G>
+struct FwdBank {
+  BandRow<W> w[G];
+  float2 u[G], sg[G], b[G], f[G];
+};
+
+template <int W, int G>
+struct BwdBank {
+  BandRow<W> w[G];
+  float2 z[G], u[G], e[G];
+};
+
+struct BandCtx {
+  const float2* Ws;  // this slice's band coefficients
+  const float2* prev;
+  float2* cur;
+  const float2* sg;
+  const float2* rb;
+  const float2* cpl;
+  int nx, ny, i, ic;  // ic = min(i, nx - 1): always-valid row for loads
+  bool act;
+};
+
+// Prefetch loads are unconditional from clamped indices (row ic = min(i, nx-1),
+// block clamped into [0, ny-1]): values of rows >= nx are never stored or
+// exchanged, and blocks outside the range are never used.
+template <int W, int G, bool SIGMA, bool RHS>
+__device__ __forceinline__ void fwd_load(FwdBank<W, G>& bk, int j0, const BandCtx& c) {
+  const f4v* w4 = reinterpret_cast<const f4v*>(c.Ws);
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int j = min(j0 + q, c.ny - 1);
+    const uint32_t o = (uint32_t)(j * c.nx + c.ic);
+    const uint32_t ow = (uint32_t)(j * (W + 1) * c.nx + c.ic);
+#pragma unroll
+    for (int t = 0; t <= W; ++t) bk.w[q].p[t] = w4[ow + t * c.nx];
+    bk.u[q] = c.prev[o];
+    if (SIGMA) bk.sg[q] = c.sg[o];
+    if (RHS) bk.b[q] = c.rb[o];
+    bk.f[q] = c.cpl[2 * j];
+  }
+}
+
+template <int W, int G>
+__device__ __forceinline__ void bwd_load(BwdBank<W, G>& bk, int j0, const BandCtx& c) {
+  const f4v* w4 = reinterpret_cast<const f4v*>(c.Ws);
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int j = max(j0 - q, 0);
+    const uint32_t o = (uint32_t)(j * c.nx + c.ic);
+    const uint32_t ow = (uint32_t)(j * (W + 1) * c.nx + c.ic);
+#pragma unroll
+    for (int t = 0; t <= W; ++t) bk.w[q].p[t] = w4[ow + t * c.nx];
+    bk.z[q] = c.cur[o];
+    bk.u[q] = c.prev[o];
+    bk.e[q] = c.cpl[2 * j + 1];
+  }
+}
+
+template <int W, int G, bool SIGMA, bool RHS>
+__device__ __forceinline__ void fwd_run(const FwdBank<W, G>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
+                                        int& xsel, float2& zprev) {
+  const float2 zero = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int j = j0 + q;
+    if (j >= c.ny) break;
+    float2 y;
+    if (SIGMA) {
+      y = cmul(bk.sg[q], bk.u[q]);
+      if (RHS) y = cadd(y, bk.b[q]);
+    } else {
+      y = RHS ? cfma(make_float2(0.5f, 0.f), bk.b[q], bk.u[q]) : bk.u[q];
+    }
+    y = cfms(bk.f[q], zprev, y);  // f_0 = 0
+    float2* xb = xbuf + xsel * XS;
+    xsel ^= 1;
+    xb[W + c.i] = c.act ? y : zero;
+    __syncthreads();
+    const float2 z = row_mv<W>(bk.w[q], xb, c.i);
+    if (c.act) c.cur[(uint32_t)(j * c.nx + c.i)] = z;
+    zprev = z;
+  }
+}
+
+template <int W, int G, bool SIGMA>
+__device__ __forceinline__ void bwd_run(const BwdBank<W, G>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
+                                        int& xsel, float2& xnext) {
+  const float2 zero = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int j = j0 - q;
+    if (j < 0) break;
+    float2 x = bk.z[q];
+    if (j < c.ny - 1) {
+      float2* xb = xbuf + xsel * XS;
+      xsel ^= 1;
+      xb[W + c.i] = c.act ? xnext : zero;
+      __syncthreads();
+      x = cfms(bk.e[q], row_mv<W>(bk.w[q], xb, c.i), bk.z[q]);
+    }
+    const float2 un = SIGMA ? csub(x, bk.u[q]) : csub(cscale(x, 2.f), bk.u[q]);
+    if (c.act) c.cur[(uint32_t)(j * c.nx + c.i)] = un;
+    xnext = x;
+  }
+}
+
+// Each thread copies its own rows of a plane (it wrote them; no other thread
+// touches them), so no synchronisation is needed.
+__device__ __forceinline__ void copy_rows(float2* dst, const float2* src, int nx, int ny, int i) {
+  for (int j = 0; j < ny; ++j) dst[j * nx + i] = src[j * nx + i];
+}
+
+template <int W, bool SIGMA, bool RHS>
 __global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) {
-  constexpr int BW = 2 * W + 1;
+  constexpr int G = BandGroup<W>::G;
   extern __shared__ float2 xbuf[];  // two exchange buffers of NT + 2W
   const int NT = blockDim.x;
   const int blk = band_xcd_block(blockIdx.x, a.nblk);
   const int sc = band_find_scan(a, blk);
-  const tp_band_scan& d = a.scans[sc];
+  const tp_band_scan d = a.scans[sc];
   const int p = blk - a.blk_start[sc];
   const int i = threadIdx.x;
   const int nx = d.nx, ny = d.ny, L = d.steps;
@@ -261,103 +399,82 @@ __global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) 
   for (int t = i; t < 2 * XS; t += NT) xbuf[t] = make_float2(0.f, 0.f);
   __syncthreads();
   int xsel = 0;
-
-  const float2* cpl = (const float2*)d.cpl;
-  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
   const float2 zero = make_float2(0.f, 0.f);
 
-  for (int s = 1; s <= L; ++s) {
-    const float2* prev = (s == 1) ? u0 : band_plane(d, p, s - 1, N);
-    float2* cur = band_plane(d, p, s, N);
-    const float2* Ws = (const float2*)d.W + (size_t)(s - 1) * ny * BW * nx;
-    const float2* sg = SIGMA ? (const float2*)d.sigma + (size_t)(s - 1) * N : nullptr;
-    const float2* rb = (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps))
-                           ? (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride
-                           : nullptr;
-    float2* out0 = (s == 1 && d.out) ? band_plane(d, p, 0, N) : nullptr;
-    float2* ck0 = (s == 1 && d.ckpt) ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride : nullptr;
+  BandCtx c;
+  c.cpl = (const float2*)d.cpl;
+  c.nx = nx;
+  c.ny = ny;
+  c.i = i;
+  c.ic = min(i, nx - 1);
+  c.act = act;
+  const size_t slice_w = (size_t)ny * (2 * W + 2) * nx;
+  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
+  if (act) {
+    if (d.out) copy_rows(band_plane(d, p, 0, N), u0, nx, ny, i);
+    if (d.ckpt) copy_rows((float2*)d.ckpt + (size_t)p * d.ckpt_pstride, u0, nx, ny, i);
+  }
 
-    // forward elimination
-    float2 wn[BW], wc[BW];
-    load_band<W>(wn, Ws, nx, i, act);
-    float2 un = act ? prev[i] : zero;
-    float2 sn = (SIGMA && act) ? sg[i] : zero;
-    float2 bn = (rb && act) ? rb[i] : zero;
-    float2 zprev = zero;
-    for (int j = 0; j < ny; ++j) {
-      const int64_t o = (int64_t)j * nx + i;
-#pragma unroll
-      for (int q = 0; q < BW; ++q) wc[q] = wn[q];
-      const float2 uc = un, sc_ = sn, bc = bn;
-      if (j + 1 < ny) {
-        load_band<W>(wn, Ws + (size_t)(j + 1) * BW * nx, nx, i, act);
-        un = act ? prev[o + nx] : zero;
-        if (SIGMA) sn = act ? sg[o + nx] : zero;
-        if (rb) bn = act ? rb[o + nx] : zero;
-      }
-      float2 y;
-      if (SIGMA) {
-        y = cmul(sc_, uc);
-        if (rb) y = cadd(y, bc);
+  for (int s = 1; s <= L; ++s) {
+    c.prev = (s == 1) ? u0 : band_plane(d, p, s - 1, N);
+    c.cur = band_plane(d, p, s, N);
+    c.Ws = (const float2*)d.W + (size_t)(s - 1) * slice_w;
+    c.sg = SIGMA ? (const float2*)d.sigma + (size_t)(s - 1) * N : nullptr;
+    c.rb = RHS ? (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride : nullptr;
+    const bool with_b = RHS && (d.rhs_steps == 0 || s <= d.rhs_steps);
+
+    // forward elimination: y_j = r_j - f_j z_{j-1}, z_j = W_j y_j (z kept in the output plane)
+    {
+      float2 zprev = zero;
+      FwdBank<W, G> A, B;
+      if (with_b) {
+        fwd_load<W, G, SIGMA, RHS>(A, 0, c);
+        for (int j0 = 0; j0 < ny; j0 += 2 * G) {
+          fwd_load<W, G, SIGMA, RHS>(B, j0 + G, c);
+          fwd_run<W, G, SIGMA, RHS>(A, j0, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, SIGMA, RHS>(A, j0 + 2 * G, c);
+          fwd_run<W, G, SIGMA, RHS>(B, j0 + G, c, xbuf, XS, xsel, zprev);
+        }
       } else {
-        y = rb ? cfma(make_float2(0.5f, 0.f), bc, uc) : uc;
+        fwd_load<W, G, SIGMA, false>(A, 0, c);
+        for (int j0 = 0; j0 < ny; j0 += 2 * G) {
+          fwd_load<W, G, SIGMA, false>(B, j0 + G, c);
+          fwd_run<W, G, SIGMA, false>(A, j0, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, SIGMA, false>(A, j0 + 2 * G, c);
+          fwd_run<W, G, SIGMA, false>(B, j0 + G, c, xbuf, XS, xsel, zprev);
+        }
       }
-      if (j > 0) y = cfms(cpl[2 * j], zprev, y);
-      float2* xb = xbuf + xsel * XS;
-      xsel ^= 1;
-      xb[W + i] = act ? y : zero;
-      __syncthreads();
-      const float2 z = band_mv<W>(wc, xb, i);
-      if (act) {
-        cur[o] = z;
-        if (out0) out0[o] = uc;
-        if (ck0) ck0[o] = uc;
-      }
-      zprev = z;
     }
 
-    // back substitution; the state update is fused into the store
-    float2* ex = (d.exitp && s == L) ? (float2*)d.exitp + (size_t)p * d.exit_pstride : nullptr;
-    float2* ck = (d.ckpt && (s % d.ckpt_every) == 0)
-                     ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride
-                     : nullptr;
-    load_band<W>(wn, Ws + (size_t)(ny - 1) * BW * nx, nx, i, act);
-    float2 zn = act ? cur[(int64_t)(ny - 1) * nx + i] : zero;
-    un = act ? prev[(int64_t)(ny - 1) * nx + i] : zero;
-    float2 xnext = zero;
-    for (int j = ny - 1; j >= 0; --j) {
-      const int64_t o = (int64_t)j * nx + i;
-#pragma unroll
-      for (int q = 0; q < BW; ++q) wc[q] = wn[q];
-      const float2 zc = zn, uc = un;
-      if (j > 0) {
-        load_band<W>(wn, Ws + (size_t)(j - 1) * BW * nx, nx, i, act);
-        zn = act ? cur[o - nx] : zero;
-        un = act ? prev[o - nx] : zero;
+    // back substitution: x_j = z_j - e_j W_j x_{j+1}; u_s = 2x - u_{s-1} (sigma: x - u_{s-1})
+    {
+      float2 xnext = zero;
+      BwdBank<W, G> A, B;
+      bwd_load<W, G>(A, ny - 1, c);
+      for (int j0 = ny - 1; j0 >= 0; j0 -= 2 * G) {
+        bwd_load<W, G>(B, j0 - G, c);
+        bwd_run<W, G, SIGMA>(A, j0, c, xbuf, XS, xsel, xnext);
+        bwd_load<W, G>(A, j0 - 2 * G, c);
+        bwd_run<W, G, SIGMA>(B, j0 - G, c, xbuf, XS, xsel, xnext);
       }
-      float2 x = zc;
-      if (j < ny - 1) {
-        float2* xb = xbuf + xsel * XS;
-        xsel ^= 1;
-        xb[W + i] = act ? xnext : zero;
-        __syncthreads();
-        x = cfms(cpl[2 * j + 1], band_mv<W>(wc, xb, i), zc);
-      }
-      const float2 un_ = SIGMA ? csub(x, uc) : csub(cscale(x, 2.f), uc);
-      if (act) {
-        cur[o] = un_;
-        if (ex) ex[o] = un_;
-        if (ck) ck[o] = un_;
-      }
-      xnext = x;
+    }
+
+    if (act) {
+      if (d.exitp && s == L) copy_rows((float2*)d.exitp + (size_t)p * d.exit_pstride, c.cur, nx, ny, i);
+      if (d.ckpt && (s % d.ckpt_every) == 0)
+        copy_rows((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, c.cur,
+                  nx, ny, i);
     }
   }
 }
 
 template <int W, bool SIGMA>
-static int launch_band_sweep(const BandSweepArgs& a, int nt, hipStream_t st) {
+static int launch_band_sweep(const BandSweepArgs& a, int nt, bool rhs, hipStream_t st) {
   const size_t lds = (size_t)2 * (nt + 2 * W) * sizeof(float2);
-  hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA>), dim3(a.nblk), dim3(nt), lds, st, a);
+  if (rhs)
+    hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA, true>), dim3(a.nblk), dim3(nt), lds, st, a);
+  else
+    hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA, false>), dim3(a.nblk), dim3(nt), lds, st, a);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
@@ -371,7 +488,7 @@ static bool band_w_supported(int w) {
 using namespace tp;
 
 extern "C" int64_t tp_band_coef_bytes(int nx, int ny, int steps, int w) {
-  return (int64_t)steps * ny * (2 * w + 1) * nx * (int64_t)sizeof(float2);
+  return (int64_t)steps * ny * (2 * w + 2) * nx * (int64_t)sizeof(float2);
 }
 
 extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const void* d_tri, const void* d_cpl,
@@ -430,6 +547,7 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream
   memset(&a, 0, sizeof(a));
   const int w = scans[0].w;
   const bool sigma = scans[0].sigma != nullptr;
+  bool rhs = false;
   int nt = 64, nblk = 0;
   for (int k = 0; k < nscans; ++k) {
     const tp_band_scan& d = scans[k];
@@ -456,6 +574,7 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream
       return TP_ERR_INVALID;
     }
     nt = std::max(nt, (d.nx + 63) / 64 * 64);
+    rhs = rhs || d.rhs != nullptr;
     a.scans[k] = d;
     a.blk_start[k] = nblk;
     nblk += d.probes;
@@ -467,7 +586,7 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream
   hipStream_t st = (hipStream_t)stream;
 #define TP_BAND_CASE(WV)                                                                           \
   case WV:                                                                                        \
-    return sigma ? launch_band_sweep<WV, true>(a, nt, st) : launch_band_sweep<WV, false>(a, nt, st);
+    return sigma ? launch_band_sweep<WV, true>(a, nt, rhs, st) : launch_band_sweep<WV, false>(a, nt, rhs, st);
   switch (w) {
     TP_BAND_CASE(4)
     TP_BAND_CASE(6)
