@@ -43,6 +43,11 @@
  *                      operator_matrices.py:75-126 (bandwidth nx).
  *   tp_band_sweep   <- _solve_single_probe_impl (iterative_lu.py:138-197) for
  *                      two transverse axes, all probes in one launch.
+ *   tp_bicgstab_solve, tp_krylov_diag, tp_cn_finish, tp_cnn_*
+ *                   <- no counterpart in the reference: an iterative
+ *                      (learned-preconditioned) alternative to the per-slice
+ *                      LU solve of iterative_lu.py:185-193 in 3D, with the
+ *                      banded factor above as its fallback.
  */
 #ifndef THICK_PTYCHO_AMD_H
 #define THICK_PTYCHO_AMD_H
@@ -250,6 +255,59 @@ typedef struct tp_band_scan {
  * w in {4, 6, 8, 12, 16, 24, 32}, nx <= 256, and the same sigma form): one workgroup per
  * probe, u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
 int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner: complex-linear 4-layer 3x3 CNN on the (ny, nx)
+ * plane of a 3D step, 1 -> 16 -> 16 -> 16 -> 1 complex channels (2 -> 32 ->
+ * 32 -> 32 -> 2 real), no biases or activations.  Parameters: complex64
+ * [tp_cnn_param_count()], layer l laid out [9 taps][OC][IC] (tap = 3*dy + dx,
+ * (dy, dx) in {0,1,2}^2 reading pixel (y+dy-1, x+dx-1)); layers in order
+ * (OC, IC) = (16,1), (16,16), (16,16), (1,16).  They are expanded into a BF16
+ * weight buffer of tp_cnn_weight_bytes() for the MFMA kernels (activations
+ * BF16, accumulation FP32).  Each plane is divided by its RMS before the
+ * network and multiplied back after it, so M is linear.
+ * ------------------------------------------------------------------------ */
+int tp_cnn_param_count(void);
+int64_t tp_cnn_weight_bytes(void);
+int64_t tp_cnn_workspace_bytes(int P, int nx, int ny, int train);
+/* d_wbuf <- BF16 expansion of d_params (caller-allocated, zeroed here). */
+int tp_cnn_expand(const void* d_params, void* d_wbuf, void* stream);
+/* d_out = M(d_in) for P planes, complex64 (P, ny*nx) row-major (flat y*nx + x). */
+int tp_cnn_apply(const void* d_wbuf, const void* d_in, void* d_out, int P, int nx, int ny, void* d_work,
+                 void* stream);
+/* One training step on pairs (in, target): loss = mean |M_n(in/s) - target/s|^2
+ * (s = RMS of each input plane) -> *d_loss (double); real weight gradients by
+ * back-propagation through the same kernels; d_grad (optional) receives the
+ * complex-parameter gradient (dL/da, dL/db per weight a + ib); with step >= 1
+ * an Adam update (lr, beta1, beta2, eps; bias correction by `step`) of
+ * d_params with moments d_m, d_v (complex64 [count], caller-zeroed before the
+ * first step) and re-expansion into d_wbuf.  step = 0 only computes. */
+int tp_cnn_train_step(void* d_params, void* d_wbuf, void* d_m, void* d_v, const void* d_in, const void* d_target,
+                      int P, int nx, int ny, int step, double lr, double beta1, double beta2, double eps,
+                      void* d_work, double* d_loss, void* d_grad, void* stream);
+
+/* ------------------------------------------------------------------------
+ * BiCGSTAB for A(s) x = b on all P planes of a slice (3D CN step)
+ * ------------------------------------------------------------------------ */
+/* d_diag (complex64 [N]) <- diagonal of A(s): tri diagonal of the block type +
+ * lhs_sign * C[:, col] (conj_c conjugates; col outside [0, ncol) adds 0).
+ * d_tri: complex128 [3][3][nx] as for tp_band_factor. */
+int tp_krylov_diag(const void* d_C, int nx, int ny, int ncol, const void* d_tri, double lhs_sign, int conj_c,
+                   int col, void* d_diag, void* stream);
+int64_t tp_bicgstab_workspace_bytes(int P, int nx, int ny, int max_iter, int precond);
+/* Right-preconditioned BiCGSTAB (precond 0 none, 1 Jacobi, 2 the CNN with
+ * weights d_cnn_w) from the initial guess in d_x; stops a plane at
+ * ||b - A x|| <= tol ||b||.  d_tri32 (complex64 [3][3][nx]) and d_cpl32
+ * (complex64 [ny][2]) hold the base operator off-diagonals.  Inner products in
+ * FP64.  d_done (int [P], optional): k >= 1 converged after k iterations, 0 not
+ * converged within max_iter, -1 breakdown.  check_every > 0 synchronises the
+ * stream every that many iterations and stops when every plane is done (not
+ * capturable); 0 runs exactly max_iter iterations without host synchronisation. */
+int tp_bicgstab_solve(const void* d_diag, const void* d_tri32, const void* d_cpl32, int nx, int ny, int P,
+                      const void* d_b, void* d_x, int precond, const void* d_cnn_w, double tol, int max_iter,
+                      int check_every, void* d_work, int* d_done, void* stream);
+/* out = xscale * x - uprev over `total` complex64 values (the CN update). */
+int tp_cn_finish(int64_t total, const void* d_x, const void* d_uprev, void* d_out, double xscale, void* stream);
 
 #ifdef __cplusplus
 }

@@ -140,6 +140,24 @@ SIGNATURES = {
          c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_void_p]),
+    "tp_cnn_param_count": (c_int, []),
+    "tp_cnn_weight_bytes": (c_int64, []),
+    "tp_cnn_workspace_bytes": (c_int64, [c_int, c_int, c_int, c_int]),
+    "tp_cnn_expand": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "tp_cnn_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "tp_cnn_train_step": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+         c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "tp_krylov_diag": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_double, c_int, c_int, c_void_p, c_void_p]),
+    "tp_bicgstab_workspace_bytes": (c_int64, [c_int, c_int, c_int, c_int, c_int]),
+    "tp_bicgstab_solve": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int,
+         c_int, c_void_p, c_void_p, c_void_p],
+    ),
+    "tp_cn_finish": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
