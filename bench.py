@@ -47,6 +47,7 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip plain-form and multislice lines")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D (config 3) banded-LU and learned lines")
     ap.add_argument("--cpu-probes", type=int, default=24)
     ap.add_argument("--ppb", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -187,6 +188,9 @@ def main() -> int:
             f"{extras['multislice_plain_value']:.4g} plain")
         del ms_out
 
+    if not args.no_3d:
+        extras["config3"] = bench_3d(dev, args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl.scans[0], args.cpu_probes)
@@ -229,6 +233,93 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_3d(dev, steps: int) -> dict:
+    """Config 3 (128 x 128 x 256, 64 probes): banded-LU pass (forward + adjoint
+    factorisation and full-field sweeps), the CNN preconditioner's MFMA rate, and
+    CN stepping with BiCGSTAB to 1e-6 (no preconditioner / trained CNN)."""
+    import numpy as np
+    import torch
+
+    from thick_ptycho_amd import learned as tl
+    from thick_ptycho_amd import workloads
+    from thick_ptycho_amd.forward_model import band_engine as be
+
+    sc = workloads.config3().scans[0]
+    P = sc.probes.shape[0]
+    nx, ny, nz = sc.nx, sc.ny, sc.nz
+    N, L = nx * ny, nz - 1
+    A, B = be.step_operators_3d(nx, ny, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
+    C = tl._contribution(sc.n, sc.k, sc.dz, dev)
+    u0 = torch.from_numpy(sc.probes.reshape(P, N).astype(np.complex64)).to(dev)
+    out_f = torch.empty((P, nz, N), dtype=torch.complex64, device=dev)
+    out_a = torch.empty((P, nz, N), dtype=torch.complex64, device=dev)
+    w = be.factor(C, A, B, "forward").w
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def ev(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    facs = {}
+
+    def factor_both():
+        facs["f"] = be.factor(C, A, B, "forward", w=w)
+        facs["a"] = be.factor(C, A, B, "adjoint", w=w)
+
+    def sweep_both():
+        be.sweep([be.BandScanSpec(facs["f"], u0, out=out_f)])
+        be.sweep([be.BandScanSpec(facs["a"], u0, out=out_a, flip=True)])
+
+    reps = max(1, min(steps, 3))
+    t_factor = ev(factor_both, reps)
+    t_sweep = ev(sweep_both, reps)
+    units = 2 * P * L
+    res = {
+        "workload": f"3D CN stepping {nx}x{ny}x{nz}, {P} probes, {sc.bc}, forward+adjoint, full fields",
+        "band_value": units / (t_factor + t_sweep),
+        "band_sweep_value": units / t_sweep,
+        "band_factor_ms": t_factor * 1e3,
+        "band_sweep_ms": t_sweep * 1e3,
+        "band_halfwidth": w,
+    }
+    log(f"3D banded LU: factor {t_factor * 1e3:.2f} ms, sweeps {t_sweep * 1e3:.2f} ms (fwd+adj), "
+        f"{res['band_value']:.4g} steps/s")
+    del out_f, out_a
+
+    # CNN preconditioner: MFMA rate of one application to all probes
+    cnn = tl.CNNPreconditioner(seed=0)
+    t_cnn = ev(lambda: cnn.apply(u0, nx, ny), 5)
+    flops = 39168.0 * N * P  # 2 * 9 * (2*32 + 32*32 + 32*32 + 32*2) per pixel
+    res["cnn_apply_ms"] = t_cnn * 1e3
+    res["cnn_tflops"] = flops / t_cnn / 1e12
+    res["cnn_mfma_frac_bf16_dense"] = flops / t_cnn / 2.5e15
+    log(f"CNN apply ({P} planes): {t_cnn * 1e3:.3f} ms, {res['cnn_tflops']:.1f} TFLOP/s")
+
+    # learned stepping: train on banded-LU pairs (seeds 31-32), step the config-3 object
+    objs = [workloads.ellipsoid_object_3d(nx, ny, 64, 32, np.random.default_rng(sd)) for sd in (31, 32)]
+    pairs = tl.generate_pairs(objs, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, per_slice=16, slices=16, seed=33)
+    t0 = time.perf_counter()
+    losses = cnn.train(pairs, steps=300, batch=16, lr=2e-3)
+    torch.cuda.synchronize()
+    res["cnn_train_s"] = time.perf_counter() - t0
+    res["cnn_train_loss"] = losses[-1]
+    for pc in ("none", "cnn"):
+        ks = tl.KrylovStepper(sc.n, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, precond=pc, cnn=cnn, tol=1e-6)
+        _, st = ks.sweep(u0)
+        res[f"krylov_{pc}_value"] = P * L / st.seconds
+        res[f"krylov_{pc}_iterations"] = float(np.mean(st.iterations))
+        res[f"krylov_{pc}_fallbacks"] = st.fallback_probes
+        log(f"3D BiCGSTAB ({pc}): {st.seconds * 1e3:.1f} ms forward sweep, mean iterations "
+            f"{res[f'krylov_{pc}_iterations']:.2f}, fallbacks {st.fallback_probes}")
+    return res
 
 
 def cpu_baseline(sc, probes: int) -> dict:

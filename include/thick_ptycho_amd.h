@@ -223,7 +223,7 @@ int64_t tp_band_coef_bytes(int nx, int ny, int steps, int w);
  * conjugates).  with_sigma != 0 also stores d_sigma complex64 [steps][N],
  * sigma = 2 + lhs_sign * c_l(s) + rhs_sign * c_r(s), c_r from rhs_col0 (the
  * caller guarantees that the RHS base operator is 2I minus the LHS base).
- * 1 <= w <= 32, w < nx, nx * (2w+1) * 16 bytes <= 160 KiB.  One wavefront per slice.
+ * w in {4, 6, 8, 12, 16, 24, 32}, w < nx, nx * (2w+1) * 16 bytes <= 160 KiB.  One wavefront per slice.
  * d_edge: float32 [steps]; d_status: caller-zeroed int, set to 1 on a zero pivot. */
 int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const void* d_tri, const void* d_cpl,
                    double lhs_sign, double rhs_sign, int conj_c, int lhs_col0, int rhs_col0, int col_step,

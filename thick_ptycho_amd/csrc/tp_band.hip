@@ -51,11 +51,14 @@ struct BandFactorArgs {
 
 __device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 
+template <int W>
 __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f) {
-  extern __shared__ double2 S[];  // band of the current block: S[i * BW + d] = M(i, i + d - w)
+  constexpr int BW = 2 * W + 1;
+  constexpr int UPD = (W * W + 63) / 64;  // LDU updates per lane
+  extern __shared__ double2 S[];          // band of the current block: S[i * BW + d] = M(i, i + d - W)
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
-  const int nx = f.nx, ny = f.ny, w = f.w, BW = 2 * w + 1;
+  const int nx = f.nx, ny = f.ny;
   const int lc = f.lcol0 + f.cstep * s;
   const int rc = f.rcol0 + f.cstep * s;
   const bool lok = lc >= 0 && lc < f.ncol;
@@ -71,20 +74,20 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
     const double2 mfe = make_double2(-fe.x, -fe.y);
     // S_j = A_jj - f_j e_{j-1} W_{j-1}  (W_{j-1} is still in S)
     for (int t = lane; t < nx * BW; t += 64) {
-      const int i = t / BW, d = t - i * BW, col = i + d - w;
+      const int i = t / BW, d = t - i * BW, col = i + d - W;
       double2 v = zero;
       if (col >= 0 && col < nx) {
         if (j > 0) v = zmul(mfe, S[t]);
-        if (d == w - 1) {
+        if (d == W - 1) {
           v = zadd(v, tri[i]);
-        } else if (d == w) {
+        } else if (d == W) {
           v = zadd(v, tri[nx + i]);
           if (lok) {
             double2 c = f.C[(size_t)(j * nx + i) * f.ncol + lc];
             if (f.conj_c) c = zconj(c);
             v = zadd(v, zscale(c, f.lsign));
           }
-        } else if (d == w + 1) {
+        } else if (d == W + 1) {
           v = zadd(v, tri[2 * nx + i]);
         }
       }
@@ -94,71 +97,73 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
 
     // banded LDU, no pivoting (the blocks are strictly diagonally dominant)
     for (int k = 0; k < nx; ++k) {
-      const int m = min(w, nx - 1 - k);
-      const double2 piv = S[k * BW + w];
+      const int m = min(W, nx - 1 - k);
+      const double2 piv = S[k * BW + W];
       if (piv.x == 0.0 && piv.y == 0.0) singular = true;
       const double2 inv = zinv(piv);
-      for (int t = lane; t < m * m; t += 64) {
-        const int a = t / m, b = t - a * m;
-        const int i = k + 1 + a, c = k + 1 + b;
-        const double2 sik = S[i * BW + (k - i + w)];
-        const double2 skc = S[k * BW + (c - k + w)];
-        const int o = i * BW + (c - i + w);
-        S[o] = zsub(S[o], zmul(zmul(sik, skc), inv));
+#pragma unroll
+      for (int u = 0; u < UPD; ++u) {
+        const int t = lane + 64 * u;
+        const int a = t / W, b = t - a * W;
+        if (t < W * W && a < m && b < m) {
+          const int i = k + 1 + a, c = k + 1 + b;
+          const double2 sik = S[i * BW + (k - i + W)];
+          const double2 skc = S[k * BW + (c - k + W)];
+          const int o = i * BW + (c - i + W);
+          S[o] = zsub(S[o], zmul(zmul(sik, skc), inv));
+        }
       }
       __syncthreads();
       if (lane < m) {
-        const int o = (k + 1 + lane) * BW + (w - 1 - lane);  // L(k+1+lane, k)
+        const int o = (k + 1 + lane) * BW + (W - 1 - lane);  // L(k+1+lane, k)
         S[o] = zmul(S[o], inv);
-      } else if (lane < 2 * m) {
-        const int o = k * BW + (w + 1 + lane - m);  // U(k, k+1+lane-m)
+      } else if (lane >= W && lane - W < m) {
+        const int o = k * BW + (W + 1 + lane - W);  // U(k, k+1+lane-W)
         S[o] = zmul(S[o], inv);
       }
       __syncthreads();
     }
 
-    // Takahashi: band of Z = S^{-1}, rows from the bottom
+    // Takahashi: band of Z = S^{-1}, rows from the bottom.  Lanes [0, W): upper
+    // entries Z(i, i+1+lane); lanes [W, 2W): lower entries Z(i+1+lane-W, i).
     for (int i = nx - 1; i >= 0; --i) {
-      const int m = min(w, nx - 1 - i);
-      double2 val = zero, prod = zero;
-      if (lane < m) {  // Z(i, c) = -sum_r U(i, r) Z(r, c)
-        const int c = i + 1 + lane;
-        double2 acc = zero;
-        for (int q = 0; q < m; ++q) {
+      const int m = min(W, nx - 1 - i);
+      const bool up = lane < W, lo = lane >= W && lane < 2 * W;
+      const int c = i + 1 + (up ? lane : lane - W);
+      const bool act = (up || lo) && c - i - 1 < m;
+      double2 acc = zero;
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        if (act && q < m) {
           const int r = i + 1 + q;
-          acc = zadd(acc, zmul(S[i * BW + (r - i + w)], S[r * BW + (c - r + w)]));
+          // upper: U(i, r) Z(r, c); lower: Z(c, r) L(r, i)
+          const double2 x = up ? S[i * BW + (r - i + W)] : S[c * BW + (r - c + W)];
+          const double2 y = up ? S[r * BW + (c - r + W)] : S[r * BW + (i - r + W)];
+          acc = zadd(acc, zmul(x, y));
         }
-        val = make_double2(-acc.x, -acc.y);
-      } else if (lane < 2 * m) {  // Z(c, i) = -sum_r Z(c, r) L(r, i)
-        const int c = i + 1 + lane - m;
-        double2 acc = zero;
-        for (int q = 0; q < m; ++q) {
-          const int r = i + 1 + q;
-          acc = zadd(acc, zmul(S[c * BW + (r - c + w)], S[r * BW + (i - r + w)]));
-        }
-        val = make_double2(-acc.x, -acc.y);
-        prod = zmul(S[i * BW + (c - i + w)], val);  // U(i, c) Z(c, i)
       }
+      const double2 val = make_double2(-acc.x, -acc.y);
+      double2 prod = zero;
+      if (act && lo) prod = zmul(S[i * BW + (c - i + W)], val);  // U(i, c) Z(c, i)
       double px = prod.x, py = prod.y;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         px += __shfl_xor(px, off, 64);
         py += __shfl_xor(py, off, 64);
       }
-      const double2 Di = S[i * BW + w];
+      const double2 Di = S[i * BW + W];
       __syncthreads();
-      if (lane < m) {
-        const int c = i + 1 + lane;
-        S[i * BW + (c - i + w)] = val;
-        if (lane == w - 1) edge = fmax(edge, zabs(val));
-      } else if (lane < 2 * m) {
-        const int c = i + 1 + lane - m;
-        S[c * BW + (i - c + w)] = val;
-        if (lane - m == w - 1) edge = fmax(edge, zabs(val));
+      if (act) {
+        if (up) {
+          S[i * BW + (c - i + W)] = val;
+        } else {
+          S[c * BW + (i - c + W)] = val;
+        }
+        if (c - i == W) edge = fmax(edge, zabs(val));
       }
       if (lane == 0) {
         const double2 z = zsub(zinv(Di), make_double2(px, py));
-        S[i * BW + w] = z;
+        S[i * BW + W] = z;
         dmax = fmax(dmax, zabs(z));
       }
       __syncthreads();
@@ -500,8 +505,8 @@ extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const v
     return TP_ERR_INVALID;
   }
   if (nx < 2 || ny < 2 || steps < 1 || ncol < 1 || w < 1 || w > 32 || w >= nx) {
-    set_error("tp_band_factor: invalid sizes (nx=%d, ny=%d, steps=%d, ncol=%d, w=%d; need nx, ny >= 2, 1 <= w <= 32, "
-              "w < nx)", nx, ny, steps, ncol, w);
+    set_error("tp_band_factor: invalid sizes (nx=%d, ny=%d, steps=%d, ncol=%d, w=%d; need nx, ny >= 2, "
+              "w in {4, 6, 8, 12, 16, 24, 32}, w < nx)", nx, ny, steps, ncol, w);
     return TP_ERR_INVALID;
   }
   const size_t lds = (size_t)nx * (2 * w + 1) * sizeof(double2);
@@ -529,11 +534,27 @@ extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const v
   f.sigma = (float2*)d_sigma;
   f.edge = d_edge;
   f.status = d_status;
-  if (lds > 65536) {
-    TP_CHECK_HIP(hipFuncSetAttribute((const void*)band_factor_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
+  hipStream_t st = (hipStream_t)stream;
+#define TP_FACTOR_W(WV)                                                                                \
+  case WV:                                                                                            \
+    if (lds > 65536)                                                                                  \
+      TP_CHECK_HIP(hipFuncSetAttribute((const void*)band_factor_kernel<WV>,                            \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));         \
+    hipLaunchKernelGGL((band_factor_kernel<WV>), dim3(steps), dim3(64), lds, st, f);                  \
+    break;
+  switch (w) {
+    TP_FACTOR_W(4)
+    TP_FACTOR_W(6)
+    TP_FACTOR_W(8)
+    TP_FACTOR_W(12)
+    TP_FACTOR_W(16)
+    TP_FACTOR_W(24)
+    TP_FACTOR_W(32)
+    default:
+      set_error("tp_band_factor: band half-width %d not built (4, 6, 8, 12, 16, 24, 32)", w);
+      return TP_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL(band_factor_kernel, dim3(steps), dim3(64), lds, (hipStream_t)stream, f);
+#undef TP_FACTOR_W
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }

@@ -260,16 +260,25 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(const WgradArgs a) {
   }
 }
 
+// workgroup sum (256 threads) -> one FP64 atomic per workgroup
+__device__ __forceinline__ void wg_atomic_add(double s, double* dst) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(dst, red[0] + red[1] + red[2] + red[3]);
+}
+
 // complex64 planes -> BF16 NHWC (re, im) divided by the plane's RMS (kept in scale)
-__global__ void cnn_norm_kernel(const float2* __restrict__ v, int64_t N, double* __restrict__ nrm2) {
+__global__ __launch_bounds__(256) void cnn_norm_kernel(const float2* __restrict__ v, int64_t N,
+                                                       double* __restrict__ nrm2) {
   const int p = blockIdx.y;
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     const float2 x = v[(size_t)p * N + i];
     s += (double)x.x * x.x + (double)x.y * x.y;
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(&nrm2[p], s);
+  wg_atomic_add(s, &nrm2[p]);
 }
 
 __global__ void cnn_pack_kernel(const float2* __restrict__ v, int64_t N, const double* __restrict__ nrm2,
@@ -288,8 +297,9 @@ __global__ void cnn_pack_kernel(const float2* __restrict__ v, int64_t N, const d
 
 // out = M(v) (already multiplied back by s): with d = (out - tgt)/s, the raw network
 // output minus tgt/s, loss += sum |d|^2 / (P N) and dout = 2 d / (P N) as BF16 NHWC (2 channels)
-__global__ void cnn_loss_kernel(const float2* __restrict__ out, const float2* __restrict__ tgt, int64_t N, int P,
-                                const float* __restrict__ scale, bf16* __restrict__ dout, double* __restrict__ loss) {
+__global__ __launch_bounds__(256) void cnn_loss_kernel(const float2* __restrict__ out, const float2* __restrict__ tgt,
+                                                       int64_t N, int P, const float* __restrict__ scale,
+                                                       bf16* __restrict__ dout, double* __restrict__ loss) {
   const int p = blockIdx.y;
   const float inv = 1.f / scale[p];
   const double norm = 1.0 / ((double)P * (double)N);
@@ -301,8 +311,7 @@ __global__ void cnn_loss_kernel(const float2* __restrict__ out, const float2* __
     dout[o * 2] = (bf16)(2.f * (float)norm * d.x);
     dout[o * 2 + 1] = (bf16)(2.f * (float)norm * d.y);
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * norm);
+  wg_atomic_add(s * norm, loss);
 }
 
 // real weight (o_r, i_r) of complex weight w: [[a, -b], [b, a]]
@@ -476,7 +485,7 @@ static void cnn_forward(const bf16* wb, const float2* in, float2* out, int P, in
                         hipStream_t st) {
   const int64_t N = (int64_t)nx * ny;
   zero_words(w.nrm2, (size_t)P * sizeof(double), st);
-  const dim3 g(N / 256 + 1 < 64 ? (unsigned)(N / 256 + 1) : 64u, (unsigned)P);
+  const dim3 g(N / 256 + 1 < 16 ? (unsigned)(N / 256 + 1) : 16u, (unsigned)P);
   hipLaunchKernelGGL(cnn_norm_kernel, g, dim3(256), 0, st, in, N, w.nrm2);
   hipLaunchKernelGGL(cnn_pack_kernel, g, dim3(256), 0, st, in, N, w.nrm2, w.x0, w.scale);
   launch_conv<2, 32, false>(w.x0, wb + OFF_W1, w.x1, nullptr, nullptr, P, nx, ny, st);
@@ -561,7 +570,7 @@ extern "C" int tp_cnn_train_step(void* d_params, void* d_wbuf, void* d_m, void* 
   const bf16* wb = (const bf16*)d_wbuf;
   cnn_forward(wb, (const float2*)d_in, w.out, P, nx, ny, w, st);
   zero_words(d_loss, sizeof(double), st);
-  const dim3 g(N / 256 + 1 < 64 ? (unsigned)(N / 256 + 1) : 64u, (unsigned)P);
+  const dim3 g(N / 256 + 1 < 16 ? (unsigned)(N / 256 + 1) : 16u, (unsigned)P);
   hipLaunchKernelGGL(cnn_loss_kernel, g, dim3(256), 0, st, w.out, (const float2*)d_target, N, P, w.scale, w.dl,
                      d_loss);
   zero_words(w.gr, (size_t)NGRAD * sizeof(float), st);
