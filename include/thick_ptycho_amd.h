@@ -153,7 +153,7 @@ int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int probes_per
 int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int s, void* stream);
 
 /* ------------------------------------------------------------------------
- * Multislice (angular spectrum), n = 64 * r with r in {4, 8, 16, 32}
+ * Multislice (angular spectrum) and far field, n = 64 * r with r = 1 .. 32
  * ------------------------------------------------------------------------ */
 
 /* T[z][x] = exp(i k (n[x, z] - n_medium) dz) for z = 0..nz-2, complex64.

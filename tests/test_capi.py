@@ -19,7 +19,7 @@ def _declared_functions():
             continue
         text = open(os.path.join(inc, fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:int|size_t|const char\*)\s+(tp_\w+)\s*\(", text, flags=re.M):
+        for m in re.finditer(r"^\s*(?:int|int64_t|size_t|const char\*)\s+(tp_\w+)\s*\(", text, flags=re.M):
             names.add(m.group(1))
     return names
 
@@ -52,3 +52,23 @@ def test_abi_version_and_error_path():
 def test_coef_bytes():
     lib = _lib.load()
     assert lib.tp_tri_coef_bytes(1000, 7, 16) == 7 * 6 * 1024 * 8
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Argument checks run before any device work (no GPU needed)."""
+    lib = _lib.load()
+    # far field: widths must be multiples of 64 up to 2048
+    assert lib.tp_farfield(1, 100, 100, 1, 1, None) == _lib.TP_ERR_UNSUPPORTED
+    assert b"multiples of 64" in lib.tp_last_error()
+    assert lib.tp_amplitude_seed(1, 1, 4096, 1, 1.0, 1, 1, None) == _lib.TP_ERR_UNSUPPORTED
+    # 3D factor: band half-width and sizes
+    assert lib.tp_band_factor(1, 64, 64, 8, 1, 1, -1.0, 1.0, 0, 0, 0, 1, 8, 5, 0, 1, None, 1, 1, None) \
+        == _lib.TP_ERR_UNSUPPORTED
+    assert lib.tp_band_factor(1, 1, 64, 8, 1, 1, -1.0, 1.0, 0, 0, 0, 1, 8, 6, 0, 1, None, 1, 1, None) \
+        == _lib.TP_ERR_INVALID
+    assert lib.tp_band_sweep(None, 0, None) == _lib.TP_ERR_INVALID
+    # Krylov / CNN
+    assert lib.tp_bicgstab_solve(1, 1, 1, 8, 8, 2, 1, 1, 3, None, 1e-6, 10, 0, 1, None, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_cnn_apply(1, 1, 1, 0, 8, 8, 1, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_cnn_param_count() == 9 * (16 + 256 + 256 + 16)
+    assert lib.tp_band_coef_bytes(128, 128, 255, 6) == 255 * 128 * 14 * 128 * 8

@@ -158,7 +158,7 @@ def test_config2_gradient(gpu):
 def test_amplitude_seed_kernel(gpu):
     """tp_amplitude_seed against the FP64 seed and loss of the same (complex64) exit waves."""
     rng = np.random.default_rng(17)
-    for n in (256, 1024):
+    for n in (64, 256, 384, 896, 1024):
         ex = (rng.standard_normal((9, n)) + 1j * rng.standard_normal((9, n))).astype(np.complex64)
         a = np.abs(np.fft.fft(ex.astype(complex), axis=-1)) * (1 + 0.3 * rng.standard_normal((9, n)))
         a = np.abs(a).astype(np.float32)

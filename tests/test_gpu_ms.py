@@ -17,7 +17,7 @@ def _obj(nx, nz, rng, amp=1e-3):
     return 1.0 - amp * (rng.random((nx, nz)) + 0.01j * rng.random((nx, nz)))
 
 
-@pytest.mark.parametrize("nx", [256, 512, 1024, 2048])
+@pytest.mark.parametrize("nx", [64, 128, 256, 384, 512, 640, 896, 1024, 1216, 1984, 2048])
 def test_ms_matches_oracle(gpu, nx, rng):
     print(f"case ms nx={nx}", flush=True)
     nz = 40
@@ -51,7 +51,7 @@ def test_ms_plane_wave_eigen(gpu):
 
 
 def test_farfield_matches_numpy(gpu, rng):
-    for nx in (256, 512, 1024, 2048):
+    for nx in (64, 128, 192, 256, 320, 384, 448, 512, 640, 704, 896, 1024, 1344, 1984, 2048):
         ex = rng.standard_normal((7, nx)) + 1j * rng.standard_normal((7, nx))
         got = ms_engine.farfield(torch.from_numpy(ex).to("cuda", torch.complex64)).cpu().numpy()
         ref = np.abs(np.fft.fft(ex, axis=-1)) ** 2
@@ -66,3 +66,21 @@ def test_config1_multislice(gpu):
     got = s.solve_device(n=sc.n, keep="exit")[0, 0].cpu().numpy()
     ref = oms.solve_batched(sc.n, sc.probes, sc.dx, sc.k, sc.dz, keep="exit")
     assert rel_l2(got, ref) < TOL
+
+
+def test_config1_extra_scans_farfield(gpu):
+    """Far-field intensities of config 1's extra scans (Nx = 384, 640, 896) through the drop-in solver."""
+    from thick_ptycho_amd import PWEIterativeLUSolver
+    from oracle import pwe
+
+    for sc in workloads.config1(nz=64).scans[1:]:
+        P = 16
+        probes = sc.probes[:P]
+        ss = SimulationSpace(sc.nx, sc.nz, sc.dx, sc.dz, num_probes=P)
+        s = PWEIterativeLUSolver(ss, probes[None], bc_type=sc.bc)
+        u = s.solve(n=sc.n)
+        got = s.get_farfield_intensities(u, poisson_noise=False)
+        g = pwe.Grid(dimension=2, nx=sc.nx, nz=sc.nz, dx=sc.dx, dz=sc.dz, k=sc.k)
+        ex = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), probes, keep="exit")
+        ref = np.abs(np.fft.fft(ex, axis=-1)) ** 2
+        assert rel_l2(got, ref) < TOL, sc.nx

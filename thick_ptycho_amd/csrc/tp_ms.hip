@@ -1,7 +1,7 @@
 // Angular-spectrum multislice propagation and far-field intensities.
 //
-// One wavefront owns one probe's field of n = 64 * R points; lane l keeps
-// x[l + 64 m] (m < R) in registers.  Each slice applies
+// One wavefront owns one probe's field of n = 64 * R points (R = 1..32); lane l
+// keeps x[l + 64 m] (m < R) in registers.  Each slice applies
 //     psi <- ifft(fft(psi * T_z) * H)
 // with a Stockham radix FFT: the first stage (radix R) runs on the register
 // layout, later stages exchange through a per-wavefront LDS ping-pong buffer,
@@ -172,6 +172,106 @@ __device__ __forceinline__ void build_twiddles(float2* tw, int n) {
   }
 }
 
+// ---------------------------------------------------------------- any n = 64 R
+// For R not in {4, 8, 16, 32}: n = l + 64 m, k = k1 + R k2 gives
+//   X[k1 + R k2] = sum_l w64^{l k2} [ wN^{l k1} sum_m x[l + 64 m] wR^{m k1} ]
+// an R-point DFT in registers (lane l, register m), the twiddle wN^{l k1}, then a
+// 64-point radix-2 transform across the lanes with xor shuffles (decimation in
+// frequency: natural in, bit-reversed out).  The spectrum is left with lane L,
+// register k1 holding X[k1 + R bitrev6(L)]; the inverse runs the steps backwards
+// (decimation in time: bit-reversed in, natural out) and returns natural order.
+template <int R>
+constexpr bool ms_pow2() {
+  return R == 4 || R == 8 || R == 16 || R == 32;
+}
+
+__device__ __forceinline__ int bitrev6(int x) { return (int)(__builtin_bitreverse32((unsigned)x) >> 26); }
+
+// frequency held by lane `lane`, register k1 after the forward transform
+template <int R>
+__device__ __forceinline__ int spec_index(int lane, int k1) {
+  if constexpr (ms_pow2<R>())
+    return lane + 64 * k1;
+  else
+    return k1 + R * bitrev6(lane);
+}
+
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+  return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+// y[k] = sum_m x[m] exp(SIGN 2 pi i m k / R), wR^j = W_N^{64 j} from the table
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(float2* x, const float2* tw) {
+  float2 wr[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) wr[j] = twid<SIGN>(tw, 64 * j);
+  float2 y[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float2 acc = x[0];
+#pragma unroll
+    for (int m = 1; m < R; ++m) acc = cfma(x[m], wr[(m * k) % R], acc);
+    y[k] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) x[k] = y[k];
+}
+
+template <int R>
+__device__ __forceinline__ void fft_gen_fwd(float2* x, const float2* tw, int lane) {
+  constexpr int N = 64 * R;
+  dft_small<R, -1>(x, tw);
+#pragma unroll
+  for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], twid<-1>(tw, (lane * k1) % N));
+#pragma unroll
+  for (int h = 32; h >= 1; h >>= 1) {
+    const float2 w = twid<-1>(tw, (lane & (h - 1)) * (32 / h) * R);
+    const bool bot = (lane & h) != 0;
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) {
+      const float2 p = shfl_xor2(x[k1], h);
+      x[k1] = bot ? cmul(csub(p, x[k1]), w) : cadd(x[k1], p);
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void fft_gen_inv(float2* x, const float2* tw, int lane) {
+  constexpr int N = 64 * R;
+#pragma unroll
+  for (int h = 1; h <= 32; h <<= 1) {
+    const float2 w = twid<+1>(tw, (lane & (h - 1)) * (32 / h) * R);
+    const bool bot = (lane & h) != 0;
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) {
+      const float2 p = shfl_xor2(x[k1], h);
+      x[k1] = bot ? csub(p, cmul(w, x[k1])) : cfma(w, p, x[k1]);
+    }
+  }
+#pragma unroll
+  for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], twid<+1>(tw, (lane * k1) % N));
+  dft_small<R, +1>(x, tw);
+}
+
+// forward (SIGN -1) / inverse (SIGN +1, unnormalised) transform of one wavefront's field
+template <int R, int SIGN>
+__device__ __forceinline__ void fft_any(float2* x, float2* a, float2* b, const float2* tw, int lane) {
+  if constexpr (ms_pow2<R>()) {
+    fft_wave<R, SIGN>(x, a, b, tw, lane);
+  } else if constexpr (SIGN < 0) {
+    fft_gen_fwd<R>(x, tw, lane);
+  } else {
+    fft_gen_inv<R>(x, tw, lane);
+  }
+}
+
+// LDS ping-pong buffers are needed only by the power-of-two Stockham path
+template <int R>
+constexpr int ms_bufn() {
+  return ms_pow2<R>() ? 64 * R : 1;
+}
+
 struct MsArgs {
   tp_ms_scan scans[TP_MAX_SCANS];
   int blk_start[TP_MAX_SCANS + 1];
@@ -208,17 +308,17 @@ __device__ __forceinline__ void ms_slice(float2* x, const float2* T, const float
                                          const float2* tw, int lane) {
 #pragma unroll
   for (int m = 0; m < R; ++m) x[m] = cmul(x[m], T[m]);
-  fft_wave<R, -1>(x, a, b, tw, lane);
+  fft_any<R, -1>(x, a, b, tw, lane);
 #pragma unroll
   for (int m = 0; m < R; ++m) x[m] = cmul(x[m], H[m]);
-  fft_wave<R, +1>(x, a, b, tw, lane);
+  fft_any<R, +1>(x, a, b, tw, lane);
 }
 
 template <int R>
 __global__ __launch_bounds__(64 * MS_PPB) void ms_sweep_kernel(const MsArgs a) {
   constexpr int N = 64 * R;
   __shared__ float2 tw[N];
-  __shared__ float2 buf[MS_PPB][2][N];
+  __shared__ float2 buf[MS_PPB][2][ms_bufn<R>()];
   const int blk = blockIdx.x;
   const int sc = ms_find_scan(a, blk);
   const tp_ms_scan& d = a.scans[sc];
@@ -234,7 +334,7 @@ __global__ __launch_bounds__(64 * MS_PPB) void ms_sweep_kernel(const MsArgs a) {
   const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
-    H[m] = Hg[lane + 64 * m];
+    H[m] = Hg[spec_index<R>(lane, m)];
     if (d.conj_h) H[m] = cconj(H[m]);
     x[m] = u0[lane + 64 * m];
     T[m] = Tg[lane + 64 * m];
@@ -256,7 +356,7 @@ template <int R>
 __global__ __launch_bounds__(64 * MS_PPB) void ms_step_kernel(const MsArgs a, int z) {
   constexpr int N = 64 * R;
   __shared__ float2 tw[N];
-  __shared__ float2 buf[MS_PPB][2][N];
+  __shared__ float2 buf[MS_PPB][2][ms_bufn<R>()];
   const int blk = blockIdx.x;
   const int sc = ms_find_scan(a, blk);
   const tp_ms_scan& d = a.scans[sc];
@@ -273,7 +373,7 @@ __global__ __launch_bounds__(64 * MS_PPB) void ms_step_kernel(const MsArgs a, in
   const float2* Tg = (const float2*)d.T + (size_t)z * N;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
-    H[m] = Hg[lane + 64 * m];
+    H[m] = Hg[spec_index<R>(lane, m)];
     if (d.conj_h) H[m] = cconj(H[m]);
     x[m] = src[lane + 64 * m];
     T[m] = Tg[lane + 64 * m];
@@ -288,7 +388,7 @@ __global__ __launch_bounds__(64 * MS_PPB) void farfield_kernel(const float2* __r
                                                                float* __restrict__ out) {
   constexpr int N = 64 * R;
   __shared__ float2 tw[N];
-  __shared__ float2 buf[MS_PPB][2][N];
+  __shared__ float2 buf[MS_PPB][2][ms_bufn<R>()];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int p = blockIdx.x * MS_PPB + wave;
@@ -298,9 +398,9 @@ __global__ __launch_bounds__(64 * MS_PPB) void farfield_kernel(const float2* __r
   float2 x[R];
 #pragma unroll
   for (int m = 0; m < R; ++m) x[m] = ex[(size_t)p * ld + lane + 64 * m];
-  fft_wave<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+  fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
-  for (int m = 0; m < R; ++m) out[(size_t)p * N + lane + 64 * m] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
+  for (int m = 0; m < R; ++m) out[(size_t)p * N + spec_index<R>(lane, m)] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
 }
 
 // Amplitude-misfit seed: U = fft(exit); loss = sum (|U| - a)^2;
@@ -312,7 +412,7 @@ __global__ __launch_bounds__(64 * MS_PPB) void amplitude_seed_kernel(const float
                                                                      float* __restrict__ loss) {
   constexpr int N = 64 * R;
   __shared__ float2 tw[N];
-  __shared__ float2 buf[MS_PPB][2][N];
+  __shared__ float2 buf[MS_PPB][2][ms_bufn<R>()];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int p = blockIdx.x * MS_PPB + wave;
@@ -322,18 +422,18 @@ __global__ __launch_bounds__(64 * MS_PPB) void amplitude_seed_kernel(const float
   float2 x[R];
 #pragma unroll
   for (int m = 0; m < R; ++m) x[m] = ex[(size_t)p * N + lane + 64 * m];
-  fft_wave<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+  fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
   float part = 0.f;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
-    const float a = amp[(size_t)p * N + lane + 64 * m];
+    const float a = amp[(size_t)p * N + spec_index<R>(lane, m)];
     const float mag = sqrtf(fmaf(x[m].x, x[m].x, x[m].y * x[m].y));
     const float dlt = mag - a;
     part = fmaf(dlt, dlt, part);
     const float w = mag > 0.f ? scale * dlt / mag : 0.f;
     x[m] = cscale(x[m], w);
   }
-  fft_wave<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+  fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
   for (int m = 0; m < R; ++m) seed[(size_t)p * N + lane + 64 * m] = x[m];
 #pragma unroll
@@ -370,16 +470,17 @@ __global__ void ms_kernel_kernel(int n, double dx, double k, double dz, float2* 
 }
 
 static int ms_radix(int n, int& r) {
-  switch (n) {
-    case 256: r = 4; return TP_OK;
-    case 512: r = 8; return TP_OK;
-    case 1024: r = 16; return TP_OK;
-    case 2048: r = 32; return TP_OK;
-    default:
-      set_error("multislice/far-field length n = %d not supported (256, 512, 1024, 2048)", n);
-      return TP_ERR_UNSUPPORTED;
+  if (n >= 64 && n <= 2048 && n % 64 == 0) {
+    r = n / 64;
+    return TP_OK;
   }
+  set_error("multislice/far-field length n = %d not supported (multiples of 64 up to 2048)", n);
+  return TP_ERR_UNSUPPORTED;
 }
+
+#define TP_MS_R_CASES(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
 
 static int ms_build(const tp_ms_scan* scans, int nscans, MsArgs& a, int& nblk, int& r) {
   if (nscans < 1 || nscans > TP_MAX_SCANS) {
@@ -448,10 +549,10 @@ extern "C" int tp_ms_sweep(const tp_ms_scan* scans, int nscans, void* stream) {
   if (nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
   switch (r) {
-    case 4: hipLaunchKernelGGL((ms_sweep_kernel<4>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((ms_sweep_kernel<8>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((ms_sweep_kernel<16>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
-    case 32: hipLaunchKernelGGL((ms_sweep_kernel<32>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
+#define TP_X(RV) \
+  case RV: hipLaunchKernelGGL((ms_sweep_kernel<RV>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
+    TP_MS_R_CASES(TP_X)
+#undef TP_X
   }
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
@@ -471,10 +572,10 @@ extern "C" int tp_ms_step(const tp_ms_scan* scans, int nscans, int z, void* stre
   if (nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
   switch (r) {
-    case 4: hipLaunchKernelGGL((ms_step_kernel<4>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
-    case 8: hipLaunchKernelGGL((ms_step_kernel<8>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
-    case 16: hipLaunchKernelGGL((ms_step_kernel<16>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
-    case 32: hipLaunchKernelGGL((ms_step_kernel<32>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
+#define TP_X(RV) \
+  case RV: hipLaunchKernelGGL((ms_step_kernel<RV>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a, z); break;
+    TP_MS_R_CASES(TP_X)
+#undef TP_X
   }
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
@@ -498,10 +599,10 @@ extern "C" int tp_amplitude_seed(const void* d_exit, const void* d_sqrt_i, int n
   float* ls = (float*)d_loss;
   const float sc = (float)scale;
   switch (r) {
-    case 4: hipLaunchKernelGGL((amplitude_seed_kernel<4>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
-    case 8: hipLaunchKernelGGL((amplitude_seed_kernel<8>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
-    case 16: hipLaunchKernelGGL((amplitude_seed_kernel<16>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
-    case 32: hipLaunchKernelGGL((amplitude_seed_kernel<32>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
+#define TP_X(RV) \
+  case RV: hipLaunchKernelGGL((amplitude_seed_kernel<RV>), grid, block, 0, st, ex, a, probes, sc, sd, ls); break;
+    TP_MS_R_CASES(TP_X)
+#undef TP_X
   }
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
@@ -521,10 +622,10 @@ extern "C" int tp_farfield(const void* d_exit, int64_t ld, int n, int probes, vo
   const float2* ex = (const float2*)d_exit;
   float* o = (float*)d_intensity;
   switch (r) {
-    case 4: hipLaunchKernelGGL((farfield_kernel<4>), grid, block, 0, st, ex, ld, probes, o); break;
-    case 8: hipLaunchKernelGGL((farfield_kernel<8>), grid, block, 0, st, ex, ld, probes, o); break;
-    case 16: hipLaunchKernelGGL((farfield_kernel<16>), grid, block, 0, st, ex, ld, probes, o); break;
-    case 32: hipLaunchKernelGGL((farfield_kernel<32>), grid, block, 0, st, ex, ld, probes, o); break;
+#define TP_X(RV) \
+  case RV: hipLaunchKernelGGL((farfield_kernel<RV>), grid, block, 0, st, ex, ld, probes, o); break;
+    TP_MS_R_CASES(TP_X)
+#undef TP_X
   }
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
