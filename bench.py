@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip plain-form and multislice lines")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D (config 3) banded-LU and learned lines")
     ap.add_argument("--cpu-probes", type=int, default=24)
-    ap.add_argument("--ppb", type=int, default=4)
+    ap.add_argument("--ppb", default="auto", help="probes per workgroup of the persistent sweep, or 'auto'")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
@@ -126,6 +126,30 @@ def main() -> int:
             fn()
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
+
+    tune = None
+    if args.ppb == "auto":
+        from thick_ptycho_amd import autotune
+
+        tune = autotune.tune_sweep(make_specs())
+        args.ppb = tune["ppb"]
+        log(f"autotuned probes per workgroup: {tune}")
+    else:
+        args.ppb = int(args.ppb)
+
+    # the plain form captured once into a HIP graph (all 1023 x 8 step launches)
+    graph_ms = None
+    if not args.no_extras:
+        gspecs = make_specs()
+        g = tri_engine.PlainGraph(gspecs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            g.replay()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t0) / max(1, args.steps // 2) * 1e3
+        log(f"plain form replayed from a HIP graph: {graph_ms:.3f} ms (sweeps only)")
+        del g, gspecs
 
     log(f"persistent batched form: warmup {args.warmup}, steps {args.steps}")
     t_pass = timed(one_pass, args.steps, args.warmup)
@@ -227,6 +251,8 @@ def main() -> int:
             "algorithmic_bytes": alg_bytes,
         },
         "factor_ms_per_pass": t_factor * 1e3,
+        "autotune": tune,
+        "plain_graph_sweep_ms": graph_ms,
         "sweep_only_value": steps_per_pass / t_kernel,
         **extras,
         "cpu_baseline": cpu,

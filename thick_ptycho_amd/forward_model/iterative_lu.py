@@ -80,7 +80,7 @@ class PWEIterativeLUSolver(BasePWESolver):
     solver_cache_class = PWEIterativeLUSolverCache
 
     def __init__(self, simulation_space, ptycho_probes, bc_type=BoundaryType.IMPEDANCE, results_dir="",
-                 use_logging=False, verbose=False, log=None, test_bcs=None, device=None, probes_per_block: int = 4):
+                 use_logging=False, verbose=False, log=None, test_bcs=None, device=None, probes_per_block=4):
         super().__init__(simulation_space, ptycho_probes, bc_type=bc_type, results_dir=results_dir,
                          use_logging=use_logging, verbose=verbose, log=log, test_bcs=test_bcs, device=device)
         self.pwe_finite_differences = PWEForwardModel(simulation_space, bc_type=bc_type)
@@ -164,6 +164,10 @@ class PWEIterativeLUSolver(BasePWESolver):
         elif plain:
             tri_engine.sweep_plain(specs)
         else:
+            if probes_per_block == "auto":
+                from .. import autotune
+
+                probes_per_block = autotune.tune_sweep(specs)["ppb"]
             tri_engine.sweep(specs, probes_per_block=probes_per_block)
 
     # ------------------------------------------------------------------ batched

@@ -198,3 +198,27 @@ def sweep_plain(specs: Sequence[TriScanSpec], stream=None) -> None:
     n = len(specs)
     for s in range(1, L + 1):
         _lib.check(lib.tp_tri_step(arr, n, chunk, s, sh), "tp_tri_step")
+
+
+class PlainGraph:
+    """The plain form (one ``tp_tri_step`` launch per slice for every scan) captured
+    once into a HIP graph and replayed.
+
+    The scan descriptors, and so every device pointer, are fixed at capture:
+    a replay steps the same buffers again.  New probes or a new object go into
+    the same tensors in place (``u0.copy_(...)``, a new factorisation written
+    into ``fac.coef``) between replays."""
+
+    def __init__(self, specs: Sequence[TriScanSpec]):
+        self.specs = list(specs)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # first launches outside the capture
+            sweep_plain(self.specs)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            sweep_plain(self.specs)
+
+    def replay(self) -> None:
+        self.graph.replay()
