@@ -43,6 +43,11 @@
  *                      operator_matrices.py:75-126 (bandwidth nx).
  *   tp_band_sweep   <- _solve_single_probe_impl (iterative_lu.py:138-197) for
  *                      two transverse axes, all probes in one launch.
+ *   tp_ms2d_kernel, tp_ms2d_sweep, tp_farfield2d
+ *                   <- no counterpart (the reference's multislice raises for
+ *                      two transverse axes, ms_solver.py:28-31): the 2D-FFT
+ *                      generalisation of MSForwardModelSolver and a per-probe
+ *                      2D far field.
  *   tp_bicgstab_solve, tp_krylov_diag, tp_cn_finish, tp_cnn_*
  *                   <- no counterpart in the reference: an iterative
  *                      (learned-preconditioned) alternative to the per-slice
@@ -255,6 +260,25 @@ typedef struct tp_band_scan {
  * w in {4, 6, 8, 12, 16, 24, 32}, nx <= 256, and the same sigma form): one workgroup per
  * probe, u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
 int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Two transverse axes: multislice with a 2D FFT per slice and per-probe 2D
+ * far field (the reference has neither: its multislice raises for 3D,
+ * ms_solver.py:28-31).  Planes complex64 (nx, ny) row-major, nx and ny powers
+ * of two from 64 to 512.
+ * ------------------------------------------------------------------------ */
+/* H[ia][ib] = exp(i (kz - k) dz) / (nx ny), kz = sqrt(max(k^2 - kx^2 - ky^2, 0)),
+ * kx = 2 pi fftfreq(nx, dx)[ia], ky = 2 pi fftfreq(ny, dy)[ib]; complex64 [nx][ny]. */
+int tp_ms2d_kernel(int nx, int ny, double dx, double dy, double k, double dz, void* d_H, void* stream);
+/* psi <- ifft2(fft2(psi * T_z) * H) (conj(H) when conj_h) for z = 0..steps-1 on
+ * every probe.  d_T: complex64 [steps][nx*ny] (tp_ms_transmission with
+ * n = nx*ny); d_u0: [probes][nx*ny]; d_out: full field [probes][steps+1][nx*ny]
+ * or NULL; d_exit: [probes][nx*ny] or NULL; d_work: 3 * probes * nx * ny complex64. */
+int tp_ms2d_sweep(const void* d_T, const void* d_H, const void* d_u0, void* d_out, void* d_exit, int nx, int ny,
+                  int steps, int probes, int conj_h, void* d_work, void* stream);
+/* |fft2(exit)|^2 per probe (unnormalised, no shift), float32 [probes][nx][ny];
+ * d_work: probes * nx * ny complex64. */
+int tp_farfield2d(const void* d_exit, int nx, int ny, int probes, void* d_intensity, void* d_work, void* stream);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner: complex-linear 4-layer 3x3 CNN on the (ny, nx)

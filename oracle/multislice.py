@@ -106,3 +106,34 @@ def farfield_per_probe_2d(exit_w: np.ndarray, nx: int, ny: int) -> np.ndarray:
     P = exit_w.shape[0]
     f = np.fft.fft2(exit_w.reshape(P, nx, ny), axes=(-2, -1))
     return (np.abs(f) ** 2).reshape(P, nx * ny)
+
+
+def propagation_kernel_2d(nx: int, ny: int, dx: float, dy: float, k: float, dz: float) -> np.ndarray:
+    """Two-axis form of ``propagation_kernel`` (the reference has no two-axis
+    multislice, ms_solver.py:28-31): exp(i (kz - k) dz), kz = sqrt(max(k^2 - kx^2 - ky^2, 0)),
+    shape (nx, ny) in numpy.fft.fftfreq order along both axes."""
+    kx = 2 * np.pi * np.fft.fftfreq(nx, d=dx)[:, None]
+    ky = 2 * np.pi * np.fft.fftfreq(ny, d=dy)[None, :]
+    kz = np.sqrt(np.clip(k**2 - kx**2 - ky**2, 0.0, None))
+    return np.exp(1j * (kz - k) * dz)
+
+
+def solve_batched_2d(n: np.ndarray, probes: np.ndarray, dx: float, dy: float, k: float, dz: float,
+                     n_medium: float = 1.0, keep: str = "all", conj_h: bool = False) -> np.ndarray:
+    """psi <- ifft2(fft2(psi * T_z) * H) over z < nz-1 for probes (P, nx, ny); n (nx, ny, nz).
+    Returns (P, nx, ny, nz) or the exit planes (P, nx, ny)."""
+    nx, ny, nz = n.shape
+    H = propagation_kernel_2d(nx, ny, dx, dy, k, dz)
+    if conj_h:
+        H = np.conj(H)
+    psi = np.asarray(probes, dtype=complex).reshape(-1, nx, ny).copy()
+    out = None
+    if keep == "all":
+        out = np.empty(psi.shape + (nz,), dtype=complex)
+        out[..., 0] = psi
+    for z in range(nz - 1):
+        psi = psi * transmission(n[:, :, z], k, dz, n_medium)[None]
+        psi = np.fft.ifft2(np.fft.fft2(psi, axes=(-2, -1)) * H[None], axes=(-2, -1))
+        if out is not None:
+            out[..., z + 1] = psi
+    return out if out is not None else psi

@@ -84,3 +84,38 @@ def test_config1_extra_scans_farfield(gpu):
         ex = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), probes, keep="exit")
         ref = np.abs(np.fft.fft(ex, axis=-1)) ** 2
         assert rel_l2(got, ref) < TOL, sc.nx
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(64, 64, 10), (128, 256, 8), (256, 256, 6), (512, 128, 5)])
+def test_multislice_2d_matches_oracle(gpu, nx, ny, nz, rng):
+    n = 1.0 - 1e-3 * (rng.random((nx, ny, nz)) + 0.01j * rng.random((nx, ny, nz)))
+    probes = rng.standard_normal((3, nx, ny)) + 1j * rng.standard_normal((3, nx, ny))
+    nt = torch.tensor(n, dtype=torch.complex128, device="cuda")
+    pt = torch.tensor(probes, dtype=torch.complex64, device="cuda")
+    full = ms_engine.multislice_2d(nt, pt, 2.0, 1.5, 2 * np.pi, 4.0, keep="all").cpu().numpy()
+    ex = ms_engine.multislice_2d(nt, pt, 2.0, 1.5, 2 * np.pi, 4.0).cpu().numpy()
+    ref = oms.solve_batched_2d(n, probes, 2.0, 1.5, 2 * np.pi, 4.0)
+    got = full.reshape(3, nz, nx, ny).transpose(0, 2, 3, 1)
+    assert rel_l2(got, ref) < TOL
+    assert rel_l2(ex.reshape(3, nx, ny), ref[..., -1]) < TOL
+
+
+def test_multislice_2d_backpropagation_inverts(gpu, rng):
+    """conj(H) undoes free-space propagation (|H| = 1 on every frequency)."""
+    nx = ny = 128
+    nt = torch.ones((nx, ny, 7), dtype=torch.complex128, device="cuda")
+    pt = torch.tensor(rng.standard_normal((2, nx * ny)) + 1j * rng.standard_normal((2, nx * ny)),
+                      dtype=torch.complex64, device="cuda")
+    fwd = ms_engine.multislice_2d(nt, pt, 2.0, 2.0, 2 * np.pi, 4.0)
+    back = ms_engine.multislice_2d(nt, fwd, 2.0, 2.0, 2 * np.pi, 4.0, conj_h=True)
+    assert rel_l2(back.cpu().numpy(), pt.cpu().numpy()) < 1e-5
+
+
+def test_farfield_2d_matches_numpy(gpu, rng):
+    for nx, ny in [(64, 128), (256, 256), (512, 64)]:
+        ex = rng.standard_normal((4, nx, ny)) + 1j * rng.standard_normal((4, nx, ny))
+        got = ms_engine.farfield_2d(torch.tensor(ex, dtype=torch.complex64, device="cuda"), nx, ny).cpu().numpy()
+        ref = oms.farfield_per_probe_2d(ex.reshape(4, -1), nx, ny).reshape(4, nx, ny)
+        assert rel_l2(got, ref) < TOL
+    with pytest.raises(ValueError):
+        ms_engine.farfield_2d(torch.zeros((1, 96 * 64), dtype=torch.complex64, device="cuda"), 96, 64)

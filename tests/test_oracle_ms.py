@@ -69,3 +69,32 @@ def test_farfield_parseval_and_delta():
     assert np.allclose(q9, np.abs(np.fft.fft2(ex3)) ** 2)
     pp = oms.farfield_per_probe_2d(ex3, 4, 4)
     assert np.allclose(pp[1], (np.abs(np.fft.fft2(ex3[1].reshape(4, 4))) ** 2).ravel())
+
+
+def test_multislice_2d_plane_wave_eigen():
+    """exp(i (kx x + ky y)) on the FFT grid is an eigenvector of the two-axis propagator
+    with eigenvalue exp(i (kz - k) dz); a constant object multiplies by exp(i k (n - 1) dz)."""
+    from oracle.multislice import solve_batched_2d
+
+    nx, ny, nz, dx, dy, dz, k = 32, 16, 6, 0.7, 0.9, 1.3, 2 * np.pi
+    mx, my = 5, 3
+    x = np.arange(nx)[:, None]
+    y = np.arange(ny)[None, :]
+    probe = np.exp(2j * np.pi * (mx * x / nx + my * y / ny))
+    dn = 1e-3 + 2e-5j
+    n = np.full((nx, ny, nz), 1.0 + dn)
+    u = solve_batched_2d(n, probe[None], dx, dy, k, dz)[0]
+    kx, ky = 2 * np.pi * mx / (nx * dx), 2 * np.pi * my / (ny * dy)
+    lam = np.exp(1j * (np.sqrt(k**2 - kx**2 - ky**2) - k) * dz) * np.exp(1j * k * dn * dz)
+    for z in range(nz):
+        assert np.allclose(u[:, :, z], probe * lam**z, atol=1e-12)
+
+
+def test_multislice_2d_unitary_in_free_space():
+    from oracle.multislice import solve_batched_2d
+
+    rng = np.random.default_rng(0)
+    nx, ny, nz = 16, 32, 5
+    probes = rng.standard_normal((2, nx, ny)) + 1j * rng.standard_normal((2, nx, ny))
+    u = solve_batched_2d(np.ones((nx, ny, nz)), probes, 2.0, 2.0, 2 * np.pi, 4.0, keep="exit")
+    assert np.allclose(np.linalg.norm(u.reshape(2, -1), axis=1), np.linalg.norm(probes.reshape(2, -1), axis=1))

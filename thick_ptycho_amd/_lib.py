@@ -158,6 +158,12 @@ SIGNATURES = {
          c_int, c_void_p, c_void_p, c_void_p],
     ),
     "tp_cn_finish": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
+    "tp_ms2d_kernel": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
+    "tp_ms2d_sweep": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    ),
+    "tp_farfield2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

@@ -99,3 +99,49 @@ def farfield(exit_w: torch.Tensor, stream=None) -> torch.Tensor:
         "tp_farfield",
     )
     return out
+
+
+# ---------------------------------------------------------------- two transverse axes
+def propagator_2d(nx: int, ny: int, dx: float, dy: float, k: float, dz: float, device, stream=None) -> torch.Tensor:
+    """H(kx, ky) / (nx ny), complex64 (nx, ny) in numpy.fft.fftfreq order along both axes."""
+    H = torch.empty((nx, ny), dtype=torch.complex64, device=device)
+    _lib.check(_lib.load().tp_ms2d_kernel(nx, ny, float(dx), float(dy), float(k), float(dz), _lib.ptr(H),
+                                          _lib.stream_handle(stream)), "tp_ms2d_kernel")
+    return H
+
+
+def multislice_2d(n: torch.Tensor, probes: torch.Tensor, dx: float, dy: float, k: float, dz: float,
+                  n_medium: float = 1.0, keep: str = "exit", conj_h: bool = False, H: Optional[torch.Tensor] = None,
+                  stream=None) -> torch.Tensor:
+    """Multislice with a 2D FFT per slice (the reference's rule with both transverse
+    frequencies; it has no two-axis form).  n: complex128 (nx, ny, nz) on the GPU;
+    probes: complex64 (P, nx, ny) or (P, nx*ny).  Returns (P, nx*ny) exit planes or
+    (P, nz, nx*ny) for keep='all'."""
+    assert n.is_cuda and n.dtype == torch.complex128 and n.dim() == 3
+    nx, ny, nz = n.shape
+    N = nx * ny
+    T = transmission(n.reshape(N, nz), k, dz, n_medium, stream)
+    if H is None:
+        H = propagator_2d(nx, ny, dx, dy, k, dz, n.device, stream)
+    u0 = probes.reshape(-1, N).to(torch.complex64).contiguous()
+    P = u0.shape[0]
+    work = torch.empty((3, P, N), dtype=torch.complex64, device=n.device)
+    out = torch.empty((P, nz, N), dtype=torch.complex64, device=n.device) if keep == "all" else None
+    ex = torch.empty((P, N), dtype=torch.complex64, device=n.device) if keep != "all" else None
+    _lib.check(
+        _lib.load().tp_ms2d_sweep(_lib.ptr(T), _lib.ptr(H), _lib.ptr(u0), _lib.ptr(out), _lib.ptr(ex), nx, ny,
+                                  nz - 1, P, int(conj_h), _lib.ptr(work), _lib.stream_handle(stream)),
+        "tp_ms2d_sweep",
+    )
+    return out if keep == "all" else ex
+
+
+def farfield_2d(exit_w: torch.Tensor, nx: int, ny: int, stream=None) -> torch.Tensor:
+    """|fft2(exit)|^2 per probe, float32 (P, nx, ny); exit complex64 (P, nx*ny) or (P, nx, ny)."""
+    ex = exit_w.reshape(-1, nx * ny).to(torch.complex64).contiguous()
+    P = ex.shape[0]
+    out = torch.empty((P, nx, ny), dtype=torch.float32, device=ex.device)
+    work = torch.empty((P, nx * ny), dtype=torch.complex64, device=ex.device)
+    _lib.check(_lib.load().tp_farfield2d(_lib.ptr(ex), nx, ny, P, _lib.ptr(out), _lib.ptr(work),
+                                         _lib.stream_handle(stream)), "tp_farfield2d")
+    return out
