@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip plain-form and multislice lines")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D (config 3) banded-LU and learned lines")
+    ap.add_argument("--no-recon", action="store_true", help="skip the 3D reconstruction loop (256x256x128)")
+    ap.add_argument("--recon-iters", type=int, default=50)
     ap.add_argument("--cpu-probes", type=int, default=24)
     ap.add_argument("--ppb", default="auto", help="probes per workgroup of the persistent sweep, or 'auto'")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -214,6 +216,8 @@ def main() -> int:
 
     if not args.no_3d:
         extras["config3"] = bench_3d(dev, args.steps)
+    if not args.no_recon:
+        extras["config4"] = bench_recon(dev, args.recon_iters)
 
     cpu = None
     if not args.no_cpu:
@@ -345,6 +349,97 @@ def bench_3d(dev, steps: int) -> dict:
         res[f"krylov_{pc}_fallbacks"] = st.fallback_probes
         log(f"3D BiCGSTAB ({pc}): {st.seconds * 1e3:.1f} ms forward sweep, mean iterations "
             f"{res[f'krylov_{pc}_iterations']:.2f}, fallbacks {st.fallback_probes}")
+    return res
+
+
+RECON_FIXTURE = os.path.join(ROOT, "tests", "golden", "recon3d_256x256x128_fp64.npz")
+
+
+def bench_recon(dev, iters: int) -> dict:
+    """3D reconstruction loop on 256 x 256 x 128 with 400 probes: intensities of the
+    seeded true object simulated on the GPU, start n0 = 1, ``iters`` steepest-descent
+    iterations (forward + exact adjoint banded-LU stepping of every probe, 2D far-field
+    loss seed, gradient, update; one HIP graph per iteration, step 0.5 / secant
+    Lipschitz estimate).  Reports iterations/s, probe·slice steps/s, the object error
+    against the true object and, against the FP64 CPU reconstruction fixture, the
+    object after 5 iterations at the fixture's step.  The 2D-FFT multislice of the
+    same object and probes is timed alongside."""
+    import numpy as np
+    import torch
+
+    from thick_ptycho_amd import recon3d, workloads
+    from thick_ptycho_amd.forward_model import ms_engine
+
+    sc = workloads.config4().scans[0]
+    nx, ny, nz = sc.nx, sc.ny, sc.nz
+    P, L = sc.probes.shape[0], nz - 1
+    probes = torch.from_numpy(sc.probes.reshape(P, -1).astype(np.complex64)).to(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    I = recon3d.simulate_intensity_3d(sc.n, probes, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
+    e1.record()
+    torch.cuda.synchronize()
+    res = {"workload": f"3D reconstruction {nx}x{ny}x{nz}, {P} probes, {sc.bc}, n0 = 1, steepest descent",
+           "simulate_ms": e0.elapsed_time(e1)}
+    n0 = np.ones((nx, ny, nz), dtype=complex)
+    n_true = torch.from_numpy(sc.n).to(dev)
+    rc = recon3d.Reconstruction3D(I.sqrt_(), probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
+    alpha = 0.5 / rc.lipschitz()
+    rc.capture()
+    rc.run(1, alpha=alpha, check_band=False)  # warm-up replay
+    rc.reset(n0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = rc.run(iters, alpha=alpha, graph=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    err = lambda: float(torch.linalg.vector_norm(rc.n - n_true) / torch.linalg.vector_norm(n_true - 1))  # noqa: E731
+    res.update({
+        "iterations": iters,
+        "iteration_ms": dt / iters * 1e3,
+        "iterations_per_s": iters / dt,
+        "value": 2 * P * L * iters / dt,
+        "unit": "probe·slice steps/s (forward + adjoint)",
+        "alpha": alpha,
+        "band_halfwidth": rc.w,
+        "loss_first": out["loss"][0],
+        "loss_last": out["loss"][-1],
+        "object_error_vs_truth_start": 1.0,
+        "object_error_vs_truth": err(),
+    })
+    log(f"3D reconstruction: {iters} iterations in {dt:.2f} s ({res['iteration_ms']:.1f} ms each), "
+        f"loss {out['loss'][0]:.4e} -> {out['loss'][-1]:.4e}, object error vs truth {res['object_error_vs_truth']:.4f}")
+    if os.path.exists(RECON_FIXTURE):
+        fx = np.load(RECON_FIXTURE)
+        rc.reset(n0)
+        o5 = rc.run(5, alpha=float(fx["alpha"]), graph=True)
+        n5 = rc.n.reshape(-1)[torch.from_numpy(fx["idx"]).to(dev)].cpu().numpy()
+        ref = fx["n5"]
+        res["fp64_5iter"] = {
+            "alpha": float(fx["alpha"]),
+            # ||n5 - n5_fp64|| / ||n5_fp64 - n0|| over the sample (n0 = 1): error relative to the update
+            "update_rel_l2_vs_fp64": float(np.linalg.norm(n5 - ref) / np.linalg.norm(ref - 1)),
+            "loss_rel_vs_fp64": float(np.max(np.abs(np.array(o5["loss"]) - fx["loss"]) / fx["loss"])),
+            "object_error_vs_truth_gpu": err(),
+            "object_error_vs_truth_fp64": float(fx["err5"]),
+            "sample": f"{ref.size} seeded positions of the object",
+        }
+        log(f"vs FP64 CPU reconstruction (5 iterations): {res['fp64_5iter']}")
+    del rc
+    # 2D-FFT multislice of the true object, exit planes of all probes
+    reps = 3
+    ms_engine.multislice_2d(n_true, probes, sc.dx, sc.dy, sc.k, sc.dz)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ms_engine.multislice_2d(n_true, probes, sc.dx, sc.dy, sc.k, sc.dz)
+    e1.record()
+    torch.cuda.synchronize()
+    t_ms = e0.elapsed_time(e1) / 1e3 / reps
+    res["multislice2d_ms"] = t_ms * 1e3
+    res["multislice2d_value"] = P * L / t_ms
+    log(f"2D-FFT multislice {nx}x{ny}, {P} probes, {L} slices: {t_ms * 1e3:.2f} ms, "
+        f"{res['multislice2d_value']:.4g} steps/s")
     return res
 
 

@@ -43,6 +43,10 @@
  *                      operator_matrices.py:75-126 (bandwidth nx).
  *   tp_band_sweep   <- _solve_single_probe_impl (iterative_lu.py:138-197) for
  *                      two transverse axes, all probes in one launch.
+ *   tp_grad3d_accumulate, tp_amplitude_seed2d
+ *                   <- the gradient of compute_grad_least_squares
+ *                      (pwe_reconstructor.py:100-164) for two transverse axes
+ *                      (exact adjoint, per-probe 2D far field).
  *   tp_ms2d_kernel, tp_ms2d_sweep, tp_farfield2d
  *                   <- no counterpart (the reference's multislice raises for
  *                      two transverse axes, ms_solver.py:28-31): the 2D-FFT
@@ -279,6 +283,19 @@ int tp_ms2d_sweep(const void* d_T, const void* d_H, const void* d_u0, void* d_ou
 /* |fft2(exit)|^2 per probe (unnormalised, no shift), float32 [probes][nx][ny];
  * d_work: probes * nx * ny complex64. */
 int tp_farfield2d(const void* d_exit, int nx, int ny, int probes, void* d_intensity, void* d_work, void* stream);
+
+/* Loss seed of the two-axis amplitude misfit: U = fft2(exit) per probe,
+ *   loss[p] += sum (|U| - a)^2   (caller zeroes loss, float32 [probes]),
+ *   seed[p]  = scale * unnormalised ifft2((|U| - a) U / |U|)  (scale = 1: dJ/d conj(exit)).
+ * a: float32 [probes][nx][ny]; d_work: 2 * probes * nx * ny complex64. */
+int tp_amplitude_seed2d(const void* d_exit, const void* d_sqrt_i, int nx, int ny, int probes, double scale,
+                        void* d_seed, void* d_loss, void* d_work, void* stream);
+/* G[s-1][f] += sum_p conj(lam_p[L-s+1][f]) (u_p[s][f] + u_p[s-1][f]) for s = 1..L (= steps):
+ * the per-slice gradient accumulation of two transverse axes from full fields,
+ * u (forward) and lam (exact adjoint in sweep order), both complex64
+ * [probes][steps+1][N]; G complex64 [steps][N]. */
+int tp_grad3d_accumulate(const void* d_u, const void* d_lam, int probes, int steps, int64_t N, void* d_G,
+                         void* stream);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner: complex-linear 4-layer 3x3 CNN on the (ny, nx)

@@ -125,3 +125,62 @@ def exact_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_me
     Gn[:, 1:] += GC
     Gn *= (g.dz / 4) * dcoef
     return 2 * np.conj(Gn), float(np.sum((np.abs(np.fft.fft(u[:, :, -1], axis=-1)) - np.sqrt(I_meas)) ** 2))
+
+
+# ----------------------------------------------------------------------------
+# Two transverse axes: amplitude misfit of the per-probe 2D far field and the
+# steepest-descent reconstruction step built on it (the three-dimensional
+# counterpart of compute_grad_least_squares with the exact adjoint).
+# ----------------------------------------------------------------------------
+
+
+def amplitude_loss_2d(exit_w: np.ndarray, I_meas: np.ndarray, nx: int, ny: int) -> float:
+    """J = sum (|fft2(u_L)| - sqrt(I))^2 over probes and detector pixels, planes (nx, ny)."""
+    U = np.fft.fft2(np.asarray(exit_w).reshape(-1, nx, ny), axes=(-2, -1))
+    return float(np.sum((np.abs(U) - np.sqrt(np.asarray(I_meas).reshape(U.shape))) ** 2))
+
+
+def amplitude_loss_seed_2d(exit_w: np.ndarray, I_meas: np.ndarray, nx: int, ny: int) -> np.ndarray:
+    """dJ/d conj(u_L) = nx ny ifft2((|U| - a) U / |U|), flattened (P, nx*ny)."""
+    P = exit_w.shape[0]
+    U = np.fft.fft2(np.asarray(exit_w).reshape(P, nx, ny), axes=(-2, -1))
+    mag = np.abs(U)
+    safe = np.where(mag == 0, 1.0, mag)
+    w = np.where(mag == 0, 0.0, (mag - np.sqrt(np.asarray(I_meas).reshape(U.shape))) / safe)
+    return (nx * ny * np.fft.ifft2(w * U, axes=(-2, -1))).reshape(P, nx * ny)
+
+
+def exact_gradient_3d(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_meas: np.ndarray, seed=None):
+    """Exact dJ/dRe(n) + i dJ/dIm(n), shape (nx, ny, nz), and J, for the 2D far-field
+    amplitude misfit.  ``seed`` replaces the loss seed when given."""
+    N = g.block_size
+    L = g.nz - 1
+    P = probes.shape[0]
+    fwd = pwe.build_lu_cache(g, bc_type, n, "forward")
+    u = pwe.solve_batched(g, fwd, probes.reshape(P, N), "forward")
+    if seed is None:
+        seed = amplitude_loss_seed_2d(u[:, :, -1], I_meas, g.nx, g.ny)
+    lam = exact_adjoint_states(g, bc_type, n, seed)
+    GC = np.zeros((N, L), dtype=complex)
+    for s in range(1, L + 1):
+        GC[:, s - 1] = np.sum(np.conj(lam[:, :, s]) * (u[:, :, s] + u[:, :, s - 1]), axis=0)
+    dcoef = g.k / 1j
+    Gn = np.zeros((N, g.nz), dtype=complex)
+    Gn[:, :-1] += GC
+    Gn[:, 1:] += GC
+    Gn *= (g.dz / 4) * dcoef
+    J = amplitude_loss_2d(u[:, :, -1], I_meas, g.nx, g.ny)
+    return (2 * np.conj(Gn)).reshape(g.nx, g.ny, g.nz), J
+
+
+def reconstruct_3d(g: pwe.Grid, bc_type, n0: np.ndarray, probes: np.ndarray, I_meas: np.ndarray, iters: int,
+                   step: float):
+    """Steepest descent n <- n - step * grad J(n) for ``iters`` iterations; returns
+    (n, [J before each iteration])."""
+    n = np.array(n0, dtype=complex)
+    hist = []
+    for _ in range(iters):
+        grad, J = exact_gradient_3d(g, bc_type, n, probes, I_meas)
+        hist.append(J)
+        n = n - step * grad
+    return n, hist

@@ -1,0 +1,59 @@
+"""Write the FP64 CPU reconstruction fixture for the 3D reconstruction workload.
+
+Workload: 256 x 256 x 128 object of 32 seeded Gaussian ellipsoids
+(thick_ptycho_amd.workloads.config4), 400 Gaussian probes (waist 16) on a
+20 x 20 raster with 60 % overlap, Dirichlet boundaries, measured intensities
+|fft2(u_L)|^2 of the true object, start n0 = 1.  Steepest descent
+n <- n - alpha grad J with alpha = 0.5 / L (secant Lipschitz estimate at n0,
+h = 1e-5), 5 iterations, all in FP64 through oracle.recon3d.
+
+Stored (tests/golden/recon3d_256x256x128_fp64.npz, ~100 KB): alpha, the loss
+before each iteration, the relative L2 error of n - 1 against n_true - 1
+before and after, and the FP64 object after 5 iterations at 4096 seeded
+sample positions (flat C-order indices).
+
+Run from the repository root (about 1.5 h on 8 cores, ~20 GB of memory):
+    python -m tests.golden.make_recon3d_256x256x128
+"""
+
+import os
+import sys
+import time
+
+import numpy as np
+
+from oracle import pwe
+from oracle import recon3d as orc
+from thick_ptycho_amd import workloads
+
+OUT = os.path.join(os.path.dirname(__file__), "recon3d_256x256x128_fp64.npz")
+ITERS = 5
+NSAMPLE = 4096
+
+
+def main():
+    t0 = time.time()
+
+    def log(msg):
+        print(f"[{time.time() - t0:8.1f} s] {msg}", flush=True)
+
+    sc = workloads.config4().scans[0]
+    g = pwe.Grid(dimension=3, nx=sc.nx, ny=sc.ny, nz=sc.nz, dx=sc.dx, dy=sc.dy, dz=sc.dz, k=sc.k)
+    P = sc.probes.shape[0]
+    probes = sc.probes.reshape(P, -1)
+    ex = orc.forward_exit_3d(g, sc.bc, sc.n, probes)
+    I = np.abs(np.fft.fft2(ex.reshape(P, sc.nx, sc.ny), axes=(-2, -1))) ** 2
+    log("intensities")
+    n0 = np.ones_like(sc.n)
+    alpha = orc.lipschitz_step(g, sc.bc, n0, probes, I, log=log)
+    log(f"alpha {alpha:.6e}")
+    n5, hist = orc.reconstruct_3d(g, sc.bc, n0, probes, I, ITERS, alpha, log=log)
+    idx = np.sort(np.random.default_rng(4096).choice(n5.size, NSAMPLE, replace=False))
+    err = lambda n: np.linalg.norm(n - sc.n) / np.linalg.norm(sc.n - 1)  # noqa: E731
+    np.savez_compressed(OUT, alpha=alpha, loss=np.array(hist), err0=err(n0), err5=err(n5), idx=idx,
+                        n5=n5.reshape(-1)[idx])
+    log(f"wrote {OUT}: loss {hist[0]:.6e} -> {hist[-1]:.6e}, error vs truth {err(n0):.5f} -> {err(n5):.5f}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

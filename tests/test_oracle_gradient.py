@@ -75,3 +75,36 @@ def test_reference_gradient_structure():
             gr2[s] -= (np.conj(gs) * lam[:, s]).real
             gi2[s] -= (np.conj(1j * gs) * lam[:, s]).real
     assert np.allclose(gr, gr2.ravel()) and np.allclose(gi, gi2.ravel())
+
+
+def test_exact_gradient_3d_central_differences():
+    """Two transverse axes: the oracle's exact gradient of the 2D far-field amplitude
+    misfit against FP64 central differences (h = 1e-6) at random entries."""
+    from oracle.gradient import amplitude_loss_2d, exact_gradient_3d
+
+    rng = np.random.default_rng(8)
+    nx, ny, nz = 8, 6, 5
+    g = pwe.Grid(dimension=3, nx=nx, ny=ny, nz=nz, dx=2.0, dy=2.0, dz=4.0, k=2 * np.pi)
+    n_true = 1 - (rng.uniform(0, 5e-3, (nx, ny, nz)) + 1j * rng.uniform(0, 5e-5, (nx, ny, nz)))
+    n0 = 1 - (rng.uniform(0, 2e-3, (nx, ny, nz)) + 1j * rng.uniform(0, 2e-5, (nx, ny, nz)))
+    probes = rng.standard_normal((3, nx * ny)) + 1j * rng.standard_normal((3, nx * ny))
+    u = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", n_true, "forward"), probes, keep="exit")
+    I = np.abs(np.fft.fft2(u.reshape(3, nx, ny), axes=(-2, -1))) ** 2
+    grad, J = exact_gradient_3d(g, "impedance", n0, probes, I)
+
+    def loss(n):
+        ex = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", n, "forward"), probes, keep="exit")
+        return amplitude_loss_2d(ex, I, nx, ny)
+
+    assert abs(loss(n0) - J) <= 1e-12 * J
+    h = 1e-6
+    scale = np.abs(grad).max()
+    for i in range(16):
+        idx = (rng.integers(nx), rng.integers(ny), rng.integers(nz))
+        d = 1.0 if i % 2 == 0 else 1j
+        npl, nmi = n0.copy(), n0.copy()
+        npl[idx] += h * d
+        nmi[idx] -= h * d
+        fd = (loss(npl) - loss(nmi)) / (2 * h)
+        an = grad[idx].real if i % 2 == 0 else grad[idx].imag
+        assert abs(fd - an) <= 1e-5 * scale + 1e-4 * abs(fd), (idx, fd, an)

@@ -131,6 +131,72 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
   }
 }
 
+// pass 2 of the amplitude seed: rows kr (the other axis' frequency); U = FFT(row),
+// loss[p] += sum (|U| - a)^2, y = IFFT(scale (|U| - a) U / |U|) stored transposed.
+// amp: [P][L][rows] (a at frequency (k_this, kr)).
+template <int R>
+__global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass a, const float* __restrict__ amp,
+                                                                  float scale, float* __restrict__ loss) {
+  constexpr int L = 64 * R;
+  constexpr int RPW = m2_rows<R>();
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
+  __shared__ float2 tile[RPW * L];
+  __shared__ float red[M2_WAVES];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  build_twiddles(tw, L);
+  const int groups = (a.rows + RPW - 1) / RPW;
+  const int p = blockIdx.x / groups, row0 = (blockIdx.x % groups) * RPW;
+  const int nr = min(RPW, a.rows - row0);
+  __syncthreads();
+  float part = 0.f;
+  for (int r = wave; r < nr; r += M2_WAVES) {
+    const int kr = row0 + r;
+    const float2* src = a.in + (size_t)p * a.in_pstride + (size_t)kr * L;
+    float2 x[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = src[lane + 64 * m];
+    fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const float av = amp[(size_t)p * a.rows * L + (size_t)spec_index<R>(lane, m) * a.rows + kr];
+      const float mag = sqrtf(fmaf(x[m].x, x[m].x, x[m].y * x[m].y));
+      const float d = mag - av;
+      part = fmaf(d, d, part);
+      x[m] = cscale(x[m], mag > 0.f ? scale * d / mag : 0.f);
+    }
+    fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+#pragma unroll
+    for (int m = 0; m < R; ++m) tile[r * L + lane + 64 * m] = x[m];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&loss[p], red[0] + red[1] + red[2] + red[3]);
+  store_tile_t(tile, a.out, a.out_pstride, p, a.rows, L, row0, nr, threadIdx.x, blockDim.x);
+}
+
+// G[s-1][f] += sum_p conj(lam_s[p][f]) (u_s[p][f] + u_{s-1}[p][f]), s = 1..L, with
+// u_s = u[p][s] (full forward field) and lam_s = lam[p][L - s + 1] (full exact-adjoint
+// field in sweep order).  One thread per (s, f); FP32 sum over the probes.
+__global__ void grad3d_kernel(const float2* __restrict__ u, const float2* __restrict__ lam, int P, int L, int64_t N,
+                              float2* __restrict__ G) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y + 1;
+  if (f >= N) return;
+  const int64_t ps = (int64_t)(L + 1) * N;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int p = 0; p < P; ++p) {
+    const float2 a = u[(size_t)p * ps + (size_t)s * N + f];
+    const float2 b = u[(size_t)p * ps + (size_t)(s - 1) * N + f];
+    const float2 l = lam[(size_t)p * ps + (size_t)(L - s + 1) * N + f];
+    acc = cfma(cconj(l), cadd(a, b), acc);
+  }
+  float2* g = G + (size_t)(s - 1) * N + f;
+  *g = cadd(*g, acc);
+}
+
 // pass 3: rows hold a spectrum in natural order; gather it into the FFT's
 // spectrum order, inverse transform, store natural (not transposed)
 template <int R>
@@ -278,6 +344,56 @@ extern "C" int tp_farfield2d(const void* d_exit, int nx, int ny, int probes, voi
   TP_M2_DISPATCH(ny, launch_p1<RV>(a1, st));
   Ms2dPass a2{(const float2*)d_work, nullptr, (float*)d_intensity, nullptr, nullptr, probes, ny, nx, 0, N, N};
   TP_M2_DISPATCH(nx, (launch_p2<RV, true>(a2, st)));
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_amplitude_seed2d(const void* d_exit, const void* d_sqrt_i, int nx, int ny, int probes, double scale,
+                                   void* d_seed, void* d_loss, void* d_work, void* stream) {
+  if (!m2_size_ok(nx) || !m2_size_ok(ny)) {
+    set_error("tp_amplitude_seed2d: plane %d x %d not supported (powers of two from 64 to 512)", nx, ny);
+    return TP_ERR_UNSUPPORTED;
+  }
+  if (!d_exit || !d_sqrt_i || !d_seed || !d_loss || !d_work || probes < 0) {
+    set_error("tp_amplitude_seed2d: invalid arguments");
+    return TP_ERR_INVALID;
+  }
+  if (probes == 0) return TP_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = (int64_t)nx * ny, PN = (int64_t)probes * N;
+  float2* w1 = (float2*)d_work;
+  float2* w2 = w1 + PN;
+  Ms2dPass a1{(const float2*)d_exit, w1, nullptr, nullptr, nullptr, probes, nx, ny, 0, N, N};
+  TP_M2_DISPATCH(ny, launch_p1<RV>(a1, st));
+  Ms2dPass a2{w1, w2, nullptr, nullptr, nullptr, probes, ny, nx, 0, N, N};
+  switch (nx / 64) {
+#define TP_SEED_CASE(RV)                                                                                     \
+  case RV: {                                                                                                \
+    const int groups = (ny + m2_rows<RV>() - 1) / m2_rows<RV>();                                              \
+    hipLaunchKernelGGL((ms2d_seed_kernel<RV>), dim3(probes * groups), dim3(64 * M2_WAVES), 0, st, a2,        \
+                       (const float*)d_sqrt_i, (float)scale, (float*)d_loss);                                \
+  } break;
+    TP_SEED_CASE(1)
+    TP_SEED_CASE(2)
+    TP_SEED_CASE(4)
+    TP_SEED_CASE(8)
+#undef TP_SEED_CASE
+  }
+  Ms2dPass a3{w2, (float2*)d_seed, nullptr, nullptr, nullptr, probes, nx, ny, 0, N, N};
+  TP_M2_DISPATCH(ny, launch_p3<RV>(a3, st));
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_grad3d_accumulate(const void* d_u, const void* d_lam, int probes, int steps, int64_t N, void* d_G,
+                                    void* stream) {
+  if (!d_u || !d_lam || !d_G || probes < 0 || steps < 1 || N < 1) {
+    set_error("tp_grad3d_accumulate: invalid arguments");
+    return TP_ERR_INVALID;
+  }
+  if (probes == 0) return TP_OK;
+  hipLaunchKernelGGL(grad3d_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)steps), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)d_u, (const float2*)d_lam, probes, steps, N, (float2*)d_G);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }

@@ -237,6 +237,49 @@ def factor(C: torch.Tensor, A: Block3D, B: Block3D, mode: str, w: Optional[int] 
     )
 
 
+class BandFactorizer:
+    """``tp_band_factor`` for one operator and mode into preallocated buffers with a
+    fixed band half-width: ``run(C)`` launches without host synchronisation (it can be
+    captured into a HIP graph); ``check()`` synchronises and verifies the band edge and
+    the pivots of the last run."""
+
+    def __init__(self, A: Block3D, B: Block3D, mode: str, L: int, w: int, device):
+        nx, ny = A.nx, A.ny
+        self.plan = band_mode_plan(A, B, mode, L)
+        self.plan.check_complementary()
+        self.nx, self.ny, self.L, self.w = nx, ny, L, w
+        self.tri = torch.from_numpy(np.ascontiguousarray(self.plan.lhs.tri)).to(device)
+        self.cpl = torch.from_numpy(np.ascontiguousarray(np.stack([self.plan.lhs.f, self.plan.lhs.e], axis=1))).to(device)
+        lib = _lib.load()
+        W = torch.empty(int(lib.tp_band_coef_bytes(nx, ny, L, w)) // 8, dtype=torch.complex64, device=device)
+        sigma = torch.empty((L, nx * ny), dtype=torch.complex64, device=device) if self.plan.sigma else None
+        self.edge = torch.zeros(L, dtype=torch.float32, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+        cpl32 = torch.from_numpy(np.stack([self.plan.lhs.f, self.plan.lhs.e], axis=1).astype(np.complex64)).to(device)
+        self.fac = BandFactors(W, cpl32.contiguous(), sigma, nx, ny, L, w, mode, self.plan.b_sign, float("nan"),
+                               keep=[self.tri, self.cpl])
+
+    def run(self, C: torch.Tensor) -> BandFactors:
+        pl = self.plan
+        _lib.check(
+            _lib.load().tp_band_factor(_lib.ptr(C), self.nx, self.ny, self.L, _lib.ptr(self.tri), _lib.ptr(self.cpl),
+                                       pl.lhs_sign, pl.rhs_sign, pl.conj_c, pl.lhs_col0, pl.rhs_col0, pl.col_step,
+                                       self.L, self.w, int(pl.sigma), _lib.ptr(self.fac.W), _lib.ptr(self.fac.sigma),
+                                       _lib.ptr(self.edge), _lib.ptr(self.status), _lib.stream_handle()),
+            "tp_band_factor",
+        )
+        return self.fac
+
+    def check(self) -> float:
+        if int(self.status.item()) != 0:
+            raise RuntimeError("singular per-slice factor (zero pivot) in tp_band_factor")
+        e = float(self.edge.max().item())
+        if e > EDGE_TOL:
+            raise RuntimeError(f"band half-width {self.w} too narrow for this object (edge ratio {e:.2e})")
+        self.fac.edge = e
+        return e
+
+
 @dataclass
 class BandScanSpec:
     """One batch of probes stepped through ``fac`` (planes of N = nx*ny values).

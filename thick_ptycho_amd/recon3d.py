@@ -1,0 +1,180 @@
+"""Iterative reconstruction of a 3D object from far-field amplitudes on the GPU.
+
+Each iteration evaluates J(n) = sum_p sum (|fft2(u_L^p)| - sqrt(I^p))^2 and its
+exact gradient (forward CN sweep of every probe, loss seed from the 2D far
+field, exact-adjoint sweep, per-slice accumulation
+G[s-1] = sum_p conj(lambda_s)(u_s + u_{s-1})), then takes the steepest-descent
+step n <- n - alpha grad J with alpha = 0.5 / L, L a secant estimate of the
+gradient's Lipschitz constant.  The step operators are factored per iteration
+with the banded block LU at a fixed band half-width, so an iteration contains no
+host synchronisation and can be captured once into a HIP graph and replayed.
+
+The reference reconstructs with NLCG (reconstruction/pwe_reconstructor.py:230-368,
+out of scope here); this loop exercises the same forward / adjoint / gradient
+kernels for two transverse axes.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .forward_model import band_engine as be
+
+
+class Reconstruction3D:
+    def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
+                 batch: Optional[int] = None, w: Optional[int] = None, device=None):
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.dev = dev
+        n0 = np.asarray(n0)
+        nx, ny, nz = n0.shape
+        self.nx, self.ny, self.nz, self.N, self.L = nx, ny, nz, nx * ny, nz - 1
+        self.k, self.dz = k, dz
+        P = probes.shape[0]
+        self.P = P
+        self.batch = batch or P
+        self.n = torch.tensor(n0, dtype=torch.complex128, device=dev)
+        self.probes = torch.as_tensor(probes).to(dev, torch.complex64).reshape(P, -1).contiguous()
+        self.sqrt_i = torch.as_tensor(sqrt_i).to(dev, torch.float32).reshape(P, nx, ny).contiguous()
+        A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
+        self.C = torch.empty((self.N, self.L), dtype=torch.complex128, device=dev)
+        self._contribution()
+        w = w or be.factor(self.C, A, B, "forward").w  # band half-width chosen once, at n0
+        self.w = w
+        self.fwd = be.BandFactorizer(A, B, "forward", self.L, w, dev)
+        self.adj = be.BandFactorizer(A, B, "exact_adjoint", self.L, w, dev)
+        Pb = self.batch
+        self.u = torch.empty((Pb, self.L + 1, self.N), dtype=torch.complex64, device=dev)
+        self.lam = torch.empty_like(self.u)
+        self.zero = torch.zeros((Pb, self.N), dtype=torch.complex64, device=dev)
+        self.seed = torch.empty((Pb, 1, self.N), dtype=torch.complex64, device=dev)
+        self.exit = torch.empty((Pb, self.N), dtype=torch.complex64, device=dev)
+        self.work = torch.empty((2, Pb, self.N), dtype=torch.complex64, device=dev)
+        self.loss = torch.zeros(P, dtype=torch.float32, device=dev)
+        self.G = torch.zeros((self.L, self.N), dtype=torch.complex64, device=dev)
+        self.grad = torch.zeros((self.N, nz), dtype=torch.complex128, device=dev)
+        self.alpha = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.graph = None
+
+    def _contribution(self) -> None:
+        coef = (self.k / 1j) * (self.n.reshape(self.N, self.nz) - 1)
+        self.C.copy_((self.dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2)
+
+    def gradient(self, seed: Optional[torch.Tensor] = None) -> None:
+        """self.loss (per probe) and self.grad ((N, nz), dJ/dRe n + i dJ/dIm n) at self.n;
+        asynchronous.  ``seed`` ((P, N) complex64 on the device) replaces the loss
+        seed dJ/d conj(u_L) in the adjoint sweep when given."""
+        lib = _lib.load()
+        sh = _lib.stream_handle
+        self._contribution()
+        fwd = self.fwd.run(self.C)
+        adj = self.adj.run(self.C)
+        self.G.zero_()
+        self.loss.zero_()
+        for b0 in range(0, self.P, self.batch):
+            b1 = min(b0 + self.batch, self.P)
+            nb = b1 - b0
+            u, lam = self.u[:nb], self.lam[:nb]
+            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)])
+            ex = self.exit[:nb]
+            ex.copy_(u[:, self.L])
+            _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb, 1.0,
+                                               _lib.ptr(self.seed[:nb]), _lib.ptr(self.loss[b0:b1]),
+                                               _lib.ptr(self.work), sh()), "tp_amplitude_seed2d")
+            rhs = self.seed[:nb] if seed is None else seed[b0:b1].reshape(nb, 1, self.N)
+            be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)])
+            _lib.check(lib.tp_grad3d_accumulate(_lib.ptr(u), _lib.ptr(lam), nb, self.L, self.N, _lib.ptr(self.G), sh()),
+                       "tp_grad3d_accumulate")
+        Gt = self.G.transpose(0, 1).to(torch.complex128)
+        self.grad.zero_()
+        self.grad[:, :-1] += Gt
+        self.grad[:, 1:] += Gt
+        self.grad.mul_((self.dz / 4) * (self.k / 1j)).conj_physical_().mul_(2)
+
+    def grad_object(self) -> torch.Tensor:
+        return self.grad.reshape(self.nx, self.ny, self.nz)
+
+    def _update(self) -> None:
+        self.n.sub_(self.alpha * self.grad.reshape(self.nx, self.ny, self.nz))
+
+    def lipschitz(self, h: float = 1e-5) -> float:
+        """Secant estimate ||g(n - d) - g(n)|| / ||d|| along d = h g / max|g|."""
+        self.gradient()
+        g0 = self.grad.clone()
+        d = (h / g0.abs().max()) * g0.reshape(self.nx, self.ny, self.nz)
+        n0 = self.n.clone()
+        self.n.sub_(d)
+        self.gradient()
+        Lc = float(torch.linalg.vector_norm(self.grad - g0) / torch.linalg.vector_norm(d))
+        self.n.copy_(n0)
+        return Lc
+
+    def capture(self) -> None:
+        """Capture one iteration (gradient + update) into a HIP graph."""
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        n_save = self.n.clone()
+        with torch.cuda.stream(side):  # first launches outside the capture
+            self.gradient()
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.gradient()
+            self._update()
+        self.n.copy_(n_save)  # the capture does not execute; keep n as it was
+
+    def run(self, iters: int, alpha: Optional[float] = None, graph: bool = True, check_band: bool = True) -> dict:
+        """Steepest descent; returns {'loss': [J before each iteration], 'alpha': ...}."""
+        if alpha is None:
+            alpha = 0.5 / self.lipschitz()
+        self.alpha.fill_(alpha)
+        if graph and self.graph is None:
+            self.capture()
+        hist: List[float] = []
+        for _ in range(iters):
+            if graph:
+                self.graph.replay()
+            else:
+                self.gradient()
+                self._update()
+            hist.append(float(self.loss.double().sum().item()))
+        if check_band:
+            self.fwd.check()
+            self.adj.check()
+        return {"loss": hist, "alpha": alpha}
+
+    def reset(self, n) -> None:
+        """Restart from the object ``n`` (nx, ny, nz); the captured graph stays valid."""
+        self.n.copy_(torch.as_tensor(np.asarray(n), dtype=torch.complex128))
+
+    def object(self) -> np.ndarray:
+        return self.n.cpu().numpy()
+
+
+def simulate_intensity_3d(n, probes, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
+                          batch: Optional[int] = None, device=None) -> torch.Tensor:
+    """|fft2(u_L)|^2 per probe, float32 (P, nx, ny) on the GPU: banded-LU forward
+    stepping of every probe through ``n`` (nx, ny, nz), exit planes only."""
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    n = np.asarray(n)
+    nx, ny, nz = n.shape
+    N = nx * ny
+    P = probes.shape[0]
+    batch = batch or P
+    coef = (k / 1j) * (n.reshape(N, nz) - 1)
+    C = torch.tensor((dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2, dtype=torch.complex128, device=dev)
+    A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
+    fac = be.factor(C, A, B, "forward")
+    u0 = torch.as_tensor(probes).to(dev, torch.complex64).reshape(P, N).contiguous()
+    ex = torch.empty((P, N), dtype=torch.complex64, device=dev)
+    for b0 in range(0, P, batch):
+        b1 = min(b0 + batch, P)
+        be.sweep([be.BandScanSpec(fac, u0[b0:b1], exit=ex[b0:b1])])
+    from .forward_model import ms_engine
+
+    return ms_engine.farfield_2d(ex, nx, ny)
