@@ -262,8 +262,21 @@ typedef struct tp_band_scan {
 
 /* Persistent 3D sweep over every probe of every scan (all with the same w,
  * w in {4, 6, 8, 12, 16, 24, 32}, nx <= 256, and the same sigma form): one workgroup per
- * probe, u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
-int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream);
+ * probes_per_block (1, 2 or 4) probes, which share each load of the band coefficients,
+ * u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
+int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_per_block, void* stream);
+
+/* Band factors of A(s)^H from those of A(s) (replaces a second tp_band_factor for
+ * the adjoint modes of iterative_lu.py:90-128, whose LHS is the conjugate
+ * transpose of the forward one): W'_j = W_j^H on the band; with reverse_slices,
+ * output slot a holds slice steps-1-a.  Same layout and size as tp_band_factor. */
+int tp_band_conj_transpose(const void* d_W, int nx, int ny, int steps, int w, int reverse_slices,
+                           void* d_WH, void* stream);
+
+/* The sigma planes of tp_band_factor alone (with_sigma output):
+ * sigma_s = 2 + lhs_sign C[:, lhs_col0 + s col_step] + rhs_sign C[:, rhs_col0 + s col_step]. */
+int tp_band_sigma(const void* d_C, int nx, int ny, int ncol, double lhs_sign, double rhs_sign, int conj_c,
+                  int lhs_col0, int rhs_col0, int col_step, int steps, void* d_sigma, void* stream);
 
 /* ------------------------------------------------------------------------
  * Two transverse axes: multislice with a 2D FFT per slice and per-probe 2D

@@ -4,18 +4,21 @@ Workload: 256 x 256 x 128 object of 32 seeded Gaussian ellipsoids
 (thick_ptycho_amd.workloads.config4), 400 Gaussian probes (waist 16) on a
 20 x 20 raster with 60 % overlap, Dirichlet boundaries, measured intensities
 |fft2(u_L)|^2 of the true object, start n0 = 1.  Steepest descent
-n <- n - alpha grad J with alpha = 0.5 / L (secant Lipschitz estimate at n0,
-h = 1e-5), 5 iterations, all in FP64 through oracle.recon3d.
+n <- n - alpha grad J, 5 iterations, all in FP64 through oracle.recon3d.
+alpha = 0.5 / L with L the secant Lipschitz estimate at n0 (h = 1e-5), or the
+value given by --alpha (the fixture records which: each FP64 gradient at this
+size takes ~15 min on 8 cores, and the estimate needs two).
 
 Stored (tests/golden/recon3d_256x256x128_fp64.npz, ~100 KB): alpha, the loss
 before each iteration, the relative L2 error of n - 1 against n_true - 1
 before and after, and the FP64 object after 5 iterations at 4096 seeded
 sample positions (flat C-order indices).
 
-Run from the repository root (about 1.5 h on 8 cores, ~20 GB of memory):
-    python -m tests.golden.make_recon3d_256x256x128
+Run from the repository root (about 1.5 h on 8 cores with --alpha, ~20 GB of memory):
+    python -m tests.golden.make_recon3d_256x256x128 [--alpha A]
 """
 
+import argparse
 import os
 import sys
 import time
@@ -32,6 +35,9 @@ NSAMPLE = 4096
 
 
 def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--alpha", type=float, default=None)
+    args = ap.parse_args()
     t0 = time.time()
 
     def log(msg):
@@ -45,12 +51,15 @@ def main():
     I = np.abs(np.fft.fft2(ex.reshape(P, sc.nx, sc.ny), axes=(-2, -1))) ** 2
     log("intensities")
     n0 = np.ones_like(sc.n)
-    alpha = orc.lipschitz_step(g, sc.bc, n0, probes, I, log=log)
-    log(f"alpha {alpha:.6e}")
+    alpha_src = "given"
+    alpha = args.alpha
+    if alpha is None:
+        alpha, alpha_src = orc.lipschitz_step(g, sc.bc, n0, probes, I, log=log), "fp64 secant estimate"
+    log(f"alpha {alpha:.6e} ({alpha_src})")
     n5, hist = orc.reconstruct_3d(g, sc.bc, n0, probes, I, ITERS, alpha, log=log)
     idx = np.sort(np.random.default_rng(4096).choice(n5.size, NSAMPLE, replace=False))
     err = lambda n: np.linalg.norm(n - sc.n) / np.linalg.norm(sc.n - 1)  # noqa: E731
-    np.savez_compressed(OUT, alpha=alpha, loss=np.array(hist), err0=err(n0), err5=err(n5), idx=idx,
+    np.savez_compressed(OUT, alpha=alpha, alpha_source=alpha_src, loss=np.array(hist), err0=err(n0), err5=err(n5), idx=idx,
                         n5=n5.reshape(-1)[idx])
     log(f"wrote {OUT}: loss {hist[0]:.6e} -> {hist[-1]:.6e}, error vs truth {err(n0):.5f} -> {err(n5):.5f}")
 

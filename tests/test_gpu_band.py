@@ -141,3 +141,76 @@ def test_config3_parity(gpu):
     err = rel_l2(got, ref)
     print(f"config 3 exit-wave rel-L2 {err:.2e}, band half-width {s.projection_cache[0].modes['forward'].factors.w}")
     assert err < TOL
+
+
+@pytest.mark.parametrize("ppb", [2, 4])
+@pytest.mark.parametrize("form", ["out_rhs_sigma", "exit_ckpt", "two_scans"])
+def test_probes_per_block_matches_one(gpu, ppb, form, rng):
+    """Probe groups sharing each band-coefficient load give the same planes as one
+    probe per workgroup (ragged groups: 7 and 5 probes)."""
+    dev = torch.device("cuda")
+    print(f"case ppb={ppb} form={form}", flush=True)
+
+    def build(nx, ny, nz, P, mode, bc="impedance", w=None):
+        n = _object(nx, ny, nz, rng)
+        A, B = be.step_operators_3d(nx, ny, K, 2.0, 2.0, 4.0, bc)
+        C = object_contribution(torch.tensor(n.reshape(nx * ny, nz), dtype=torch.complex128, device=dev), K, 4.0)
+        fac = be.factor(C, A, B, mode, w=w)
+        u0 = torch.tensor(_probes(P, nx, ny, rng).reshape(P, -1), dtype=torch.complex64, device=dev)
+        return fac, u0
+
+    def specs_for():
+        if form == "out_rhs_sigma":
+            fac, u0 = build(40, 24, 9, 7, "exact_adjoint")
+            rhs = torch.randn((7, 1, 40 * 24), dtype=torch.complex64, device=dev)
+            return [be.BandScanSpec(fac, u0, out=torch.empty((7, 9, 40 * 24), dtype=torch.complex64, device=dev),
+                                    rhs=rhs, rhs_steps=1)]
+        if form == "exit_ckpt":
+            fac, u0 = build(33, 20, 11, 5, "forward")
+            return [be.BandScanSpec(fac, u0, exit=torch.empty((5, 33 * 20), dtype=torch.complex64, device=dev),
+                                    ckpt=torch.empty((5, 10 // 4 + 1, 33 * 20), dtype=torch.complex64, device=dev),
+                                    ckpt_every=4)]
+        out = []
+        for nx, ny, P in [(48, 32, 7), (20, 36, 5)]:
+            fac, u0 = build(nx, ny, 8, P, "forward", w=8)
+            out.append(be.BandScanSpec(fac, u0, out=torch.empty((P, 8, nx * ny), dtype=torch.complex64, device=dev)))
+        return out
+
+    specs = specs_for()
+
+    def outputs():
+        return [t.clone() for s in specs for t in (s.out, s.exit, s.ckpt) if t is not None]
+
+    be.sweep(specs, probes_per_block=1)
+    ref = outputs()
+    for s in specs:
+        for t in (s.out, s.exit, s.ckpt):
+            if t is not None:
+                t.fill_(float("nan"))
+    be.sweep(specs, probes_per_block=ppb)
+    for a, b in zip(outputs(), ref):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann", "impedance"])
+@pytest.mark.parametrize("mode", ["adjoint", "exact_adjoint"])
+def test_adjoint_factors_from_forward(gpu, bc, mode, rng):
+    """tp_band_conj_transpose + tp_band_sigma reproduce a direct factorisation of the
+    adjoint operators (W'_j = W_j^H, slices reversed)."""
+    nx, ny, nz = 36, 28, 10
+    dev = torch.device("cuda")
+    n = _object(nx, ny, nz, rng)
+    A, B = be.step_operators_3d(nx, ny, K, 2.0, 2.0, 4.0, bc)
+    C = object_contribution(torch.tensor(n.reshape(nx * ny, nz), dtype=torch.complex128, device=dev), K, 4.0)
+    w = be.factor(C, A, B, "forward").w
+    fwd = be.BandFactorizer(A, B, "forward", nz - 1, w, dev)
+    direct = be.BandFactorizer(A, B, mode, nz - 1, w, dev)
+    derived = be.BandFactorizer(A, B, mode, nz - 1, w, dev, source=fwd)
+    fwd.run(C)
+    a, b = direct.run(C), derived.run(C)
+    direct.check()
+    derived.check()
+    scale = a.W.abs().max()
+    assert float((a.W - b.W).abs().max() / scale) < 1e-6
+    if a.sigma is not None:
+        assert torch.equal(a.sigma, b.sigma)

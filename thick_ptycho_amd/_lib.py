@@ -139,7 +139,10 @@ SIGNATURES = {
         [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int,
          c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
-    "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_void_p]),
+    "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_int, c_void_p]),
+    "tp_band_conj_transpose": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "tp_band_sigma": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int,
+                              c_void_p, c_void_p]),
     "tp_cnn_param_count": (c_int, []),
     "tp_cnn_weight_bytes": (c_int64, []),
     "tp_cnn_workspace_bytes": (c_int64, [c_int, c_int, c_int, c_int]),

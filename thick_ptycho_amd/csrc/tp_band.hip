@@ -51,6 +51,23 @@ struct BandFactorArgs {
 
 __device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 
+// sigma(t) = 2 + lsign C(t, lc) + rsign C(t, rc) (columns outside the object dropped)
+__device__ __forceinline__ float2 band_sigma_at(const double2* C, int ncol, int64_t t, int lc, bool lok, int rc, bool rok,
+                                                double lsign, double rsign, int conj_c) {
+  double2 v = make_double2(2.0, 0.0);
+  if (lok) {
+    double2 c = C[t * ncol + lc];
+    if (conj_c) c = zconj(c);
+    v = zadd(v, zscale(c, lsign));
+  }
+  if (rok) {
+    double2 c = C[t * ncol + rc];
+    if (conj_c) c = zconj(c);
+    v = zadd(v, zscale(c, rsign));
+  }
+  return to_f2(v);
+}
+
 template <int W>
 __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f) {
   constexpr int BW = 2 * W + 1;
@@ -181,20 +198,8 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
   if (f.with_sigma) {  // sigma = lhs diagonal + rhs diagonal = 2 + object terms of both slices
     const int N = nx * ny;
     float2* sg = f.sigma + (size_t)s * N;
-    for (int t = lane; t < N; t += 64) {
-      double2 v = make_double2(2.0, 0.0);
-      if (lok) {
-        double2 c = f.C[(size_t)t * f.ncol + lc];
-        if (f.conj_c) c = zconj(c);
-        v = zadd(v, zscale(c, f.lsign));
-      }
-      if (rok) {
-        double2 c = f.C[(size_t)t * f.ncol + rc];
-        if (f.conj_c) c = zconj(c);
-        v = zadd(v, zscale(c, f.rsign));
-      }
-      sg[t] = to_f2(v);
-    }
+    for (int t = lane; t < N; t += 64)
+      sg[t] = band_sigma_at(f.C, f.ncol, t, lc, lok, rc, rok, f.lsign, f.rsign, f.conj_c);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -266,42 +271,51 @@ __device__ __forceinline__ float2 row_mv(const BandRow<W>& r, const float2* xb, 
 
 // Blocks are processed in groups of G; the next group's operands are fetched
 // into a second register bank while the current group computes, so a step
-// waits only on loads issued a whole group earlier.  G keeps one group's loads
-// below the 63 outstanding vector-memory operations the wait counter tracks.
-template <int W>
+// waits only on loads issued a whole group earlier.  A workgroup steps PB
+// probes together: each thread's row of W_j is loaded once and applied to all
+// PB probes, which divides the band-coefficient traffic (the dominant HBM
+// stream when many probes share a slice) by PB.  G keeps two banks of G rows
+// within the register budget and one group's loads below the 63 outstanding
+// vector-memory operations the wait counter tracks.
+template <int W, int PB>
 struct BandGroup {
-  // two banks of G rows of W+1 16-byte pairs (+ 4 scalars) stay within 256 VGPRs
-  static constexpr int G = W <= 4 ? 4 : (W <= 6 ? 3 : (W <= 12 ? 2 : 1));
+  static constexpr int R = (W + 1) * 4 + PB * 4 + 4;  // VGPRs of one block of one bank
+  static constexpr int GP = 100 / R;
+  static constexpr int G = PB == 1 ? (W <= 4 ? 4 : (W <= 6 ? 3 : (W <= 12 ? 2 : 1)))
+                                   : (GP < 1 ? 1 : (GP > 4 ? 4 : GP));
 };
 
-template <int W, int G>
+template <int W, int G, int PB>
 struct FwdBank {
   BandRow<W> w[G];
-  float2 u[G], sg[G], b[G], f[G];
+  float2 u[G][PB], b[G][PB], sg[G], f[G];
 };
 
-template <int W, int G>
+template <int W, int G, int PB>
 struct BwdBank {
   BandRow<W> w[G];
-  float2 z[G], u[G], e[G];
+  float2 z[G][PB], u[G][PB], e[G];
 };
 
 struct BandCtx {
   const float2* Ws;  // this slice's band coefficients
-  const float2* prev;
+  const float2* prev;  // probe k's previous plane at prev + k * prev_ps
   float2* cur;
   const float2* sg;
   const float2* rb;
   const float2* cpl;
+  int64_t prev_ps, cur_ps, rb_ps;
   int nx, ny, i, ic;  // ic = min(i, nx - 1): always-valid row for loads
+  int np;             // probes of this workgroup (1 .. PB)
   bool act;
 };
 
 // Prefetch loads are unconditional from clamped indices (row ic = min(i, nx-1),
-// block clamped into [0, ny-1]): values of rows >= nx are never stored or
-// exchanged, and blocks outside the range are never used.
-template <int W, int G, bool SIGMA, bool RHS>
-__device__ __forceinline__ void fwd_load(FwdBank<W, G>& bk, int j0, const BandCtx& c) {
+// block clamped into [0, ny-1], probe clamped into [0, np-1]): values of rows
+// >= nx or of absent probes are never stored, and blocks outside the range are
+// never used.
+template <int W, int G, int PB, bool SIGMA, bool RHS>
+__device__ __forceinline__ void fwd_load(FwdBank<W, G, PB>& bk, int j0, const BandCtx& c) {
   const f4v* w4 = reinterpret_cast<const f4v*>(c.Ws);
 #pragma unroll
   for (int q = 0; q < G; ++q) {
@@ -310,15 +324,19 @@ __device__ __forceinline__ void fwd_load(FwdBank<W, G>& bk, int j0, const BandCt
     const uint32_t ow = (uint32_t)(j * (W + 1) * c.nx + c.ic);
 #pragma unroll
     for (int t = 0; t <= W; ++t) bk.w[q].p[t] = w4[ow + t * c.nx];
-    bk.u[q] = c.prev[o];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int kk = min(k, c.np - 1);
+      bk.u[q][k] = c.prev[kk * c.prev_ps + o];
+      if (RHS) bk.b[q][k] = c.rb[kk * c.rb_ps + o];
+    }
     if (SIGMA) bk.sg[q] = c.sg[o];
-    if (RHS) bk.b[q] = c.rb[o];
     bk.f[q] = c.cpl[2 * j];
   }
 }
 
-template <int W, int G>
-__device__ __forceinline__ void bwd_load(BwdBank<W, G>& bk, int j0, const BandCtx& c) {
+template <int W, int G, int PB>
+__device__ __forceinline__ void bwd_load(BwdBank<W, G, PB>& bk, int j0, const BandCtx& c) {
   const f4v* w4 = reinterpret_cast<const f4v*>(c.Ws);
 #pragma unroll
   for (int q = 0; q < G; ++q) {
@@ -327,57 +345,74 @@ __device__ __forceinline__ void bwd_load(BwdBank<W, G>& bk, int j0, const BandCt
     const uint32_t ow = (uint32_t)(j * (W + 1) * c.nx + c.ic);
 #pragma unroll
     for (int t = 0; t <= W; ++t) bk.w[q].p[t] = w4[ow + t * c.nx];
-    bk.z[q] = c.cur[o];
-    bk.u[q] = c.prev[o];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int kk = min(k, c.np - 1);
+      bk.z[q][k] = c.cur[kk * c.cur_ps + o];
+      bk.u[q][k] = c.prev[kk * c.prev_ps + o];
+    }
     bk.e[q] = c.cpl[2 * j + 1];
   }
 }
 
-template <int W, int G, bool SIGMA, bool RHS>
-__device__ __forceinline__ void fwd_run(const FwdBank<W, G>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
-                                        int& xsel, float2& zprev) {
+template <int W, int G, int PB, bool SIGMA, bool RHS>
+__device__ __forceinline__ void fwd_run(const FwdBank<W, G, PB>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
+                                        int& xsel, float2 (&zprev)[PB]) {
   const float2 zero = make_float2(0.f, 0.f);
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     const int j = j0 + q;
     if (j >= c.ny) break;
-    float2 y;
-    if (SIGMA) {
-      y = cmul(bk.sg[q], bk.u[q]);
-      if (RHS) y = cadd(y, bk.b[q]);
-    } else {
-      y = RHS ? cfma(make_float2(0.5f, 0.f), bk.b[q], bk.u[q]) : bk.u[q];
-    }
-    y = cfms(bk.f[q], zprev, y);  // f_0 = 0
-    float2* xb = xbuf + xsel * XS;
+    float2* xb = xbuf + xsel * PB * XS;
     xsel ^= 1;
-    xb[W + c.i] = c.act ? y : zero;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      float2 y;
+      if (SIGMA) {
+        y = cmul(bk.sg[q], bk.u[q][k]);
+        if (RHS) y = cadd(y, bk.b[q][k]);
+      } else {
+        y = RHS ? cfma(make_float2(0.5f, 0.f), bk.b[q][k], bk.u[q][k]) : bk.u[q][k];
+      }
+      y = cfms(bk.f[q], zprev[k], y);  // f_0 = 0
+      xb[k * XS + W + c.i] = c.act ? y : zero;
+    }
     __syncthreads();
-    const float2 z = row_mv<W>(bk.w[q], xb, c.i);
-    if (c.act) c.cur[(uint32_t)(j * c.nx + c.i)] = z;
-    zprev = z;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const float2 z = row_mv<W>(bk.w[q], xb + k * XS, c.i);
+      if (c.act && k < c.np) c.cur[k * c.cur_ps + (uint32_t)(j * c.nx + c.i)] = z;
+      zprev[k] = z;
+    }
   }
 }
 
-template <int W, int G, bool SIGMA>
-__device__ __forceinline__ void bwd_run(const BwdBank<W, G>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
-                                        int& xsel, float2& xnext) {
+template <int W, int G, int PB, bool SIGMA>
+__device__ __forceinline__ void bwd_run(const BwdBank<W, G, PB>& bk, int j0, const BandCtx& c, float2* xbuf, int XS,
+                                        int& xsel, float2 (&xnext)[PB]) {
   const float2 zero = make_float2(0.f, 0.f);
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     const int j = j0 - q;
     if (j < 0) break;
-    float2 x = bk.z[q];
+    float2 x[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) x[k] = bk.z[q][k];
     if (j < c.ny - 1) {
-      float2* xb = xbuf + xsel * XS;
+      float2* xb = xbuf + xsel * PB * XS;
       xsel ^= 1;
-      xb[W + c.i] = c.act ? xnext : zero;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) xb[k * XS + W + c.i] = c.act ? xnext[k] : zero;
       __syncthreads();
-      x = cfms(bk.e[q], row_mv<W>(bk.w[q], xb, c.i), bk.z[q]);
+#pragma unroll
+      for (int k = 0; k < PB; ++k) x[k] = cfms(bk.e[q], row_mv<W>(bk.w[q], xb + k * XS, c.i), bk.z[q][k]);
     }
-    const float2 un = SIGMA ? csub(x, bk.u[q]) : csub(cscale(x, 2.f), bk.u[q]);
-    if (c.act) c.cur[(uint32_t)(j * c.nx + c.i)] = un;
-    xnext = x;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const float2 un = SIGMA ? csub(x[k], bk.u[q][k]) : csub(cscale(x[k], 2.f), bk.u[q][k]);
+      if (c.act && k < c.np) c.cur[k * c.cur_ps + (uint32_t)(j * c.nx + c.i)] = un;
+      xnext[k] = x[k];
+    }
   }
 }
 
@@ -387,21 +422,26 @@ __device__ __forceinline__ void copy_rows(float2* dst, const float2* src, int nx
   for (int j = 0; j < ny; ++j) dst[j * nx + i] = src[j * nx + i];
 }
 
-template <int W, bool SIGMA, bool RHS>
+__device__ __forceinline__ int64_t band_plane_stride(const tp_band_scan& d) {
+  return d.out ? d.out_pstride : d.work_pstride;
+}
+
+template <int W, int PB, bool SIGMA, bool RHS>
 __global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) {
-  constexpr int G = BandGroup<W>::G;
-  extern __shared__ float2 xbuf[];  // two exchange buffers of NT + 2W
+  constexpr int G = BandGroup<W, PB>::G;
+  extern __shared__ float2 xbuf[];  // two exchange buffers of PB x (NT + 2W)
   const int NT = blockDim.x;
   const int blk = band_xcd_block(blockIdx.x, a.nblk);
   const int sc = band_find_scan(a, blk);
   const tp_band_scan d = a.scans[sc];
-  const int p = blk - a.blk_start[sc];
+  const int p0 = (blk - a.blk_start[sc]) * PB;
+  const int np = min(PB, d.probes - p0);
   const int i = threadIdx.x;
   const int nx = d.nx, ny = d.ny, L = d.steps;
   const int64_t N = (int64_t)nx * ny;
   const bool act = i < nx;
   const int XS = NT + 2 * W;
-  for (int t = i; t < 2 * XS; t += NT) xbuf[t] = make_float2(0.f, 0.f);
+  for (int t = i; t < 2 * PB * XS; t += NT) xbuf[t] = make_float2(0.f, 0.f);
   __syncthreads();
   int xsel = 0;
   const float2 zero = make_float2(0.f, 0.f);
@@ -413,79 +453,146 @@ __global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) 
   c.i = i;
   c.ic = min(i, nx - 1);
   c.act = act;
+  c.np = np;
+  c.cur_ps = band_plane_stride(d);
+  c.rb_ps = d.rhs_pstride;
   const size_t slice_w = (size_t)ny * (2 * W + 2) * nx;
-  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
+  const float2* u0 = (const float2*)d.u0 + (size_t)p0 * d.u0_pstride;
   if (act) {
-    if (d.out) copy_rows(band_plane(d, p, 0, N), u0, nx, ny, i);
-    if (d.ckpt) copy_rows((float2*)d.ckpt + (size_t)p * d.ckpt_pstride, u0, nx, ny, i);
+    for (int k = 0; k < np; ++k) {
+      const float2* src = u0 + k * d.u0_pstride;
+      if (d.out) copy_rows(band_plane(d, p0 + k, 0, N), src, nx, ny, i);
+      if (d.ckpt) copy_rows((float2*)d.ckpt + (size_t)(p0 + k) * d.ckpt_pstride, src, nx, ny, i);
+    }
   }
 
   for (int s = 1; s <= L; ++s) {
-    c.prev = (s == 1) ? u0 : band_plane(d, p, s - 1, N);
-    c.cur = band_plane(d, p, s, N);
+    c.prev = (s == 1) ? u0 : band_plane(d, p0, s - 1, N);
+    c.prev_ps = (s == 1) ? d.u0_pstride : c.cur_ps;
+    c.cur = band_plane(d, p0, s, N);
     c.Ws = (const float2*)d.W + (size_t)(s - 1) * slice_w;
     c.sg = SIGMA ? (const float2*)d.sigma + (size_t)(s - 1) * N : nullptr;
-    c.rb = RHS ? (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride : nullptr;
+    c.rb = RHS ? (const float2*)d.rhs + (size_t)p0 * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride : nullptr;
     const bool with_b = RHS && (d.rhs_steps == 0 || s <= d.rhs_steps);
 
     // forward elimination: y_j = r_j - f_j z_{j-1}, z_j = W_j y_j (z kept in the output plane)
     {
-      float2 zprev = zero;
-      FwdBank<W, G> A, B;
+      float2 zprev[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) zprev[k] = zero;
+      FwdBank<W, G, PB> A, B;
       if (with_b) {
-        fwd_load<W, G, SIGMA, RHS>(A, 0, c);
+        fwd_load<W, G, PB, SIGMA, RHS>(A, 0, c);
         for (int j0 = 0; j0 < ny; j0 += 2 * G) {
-          fwd_load<W, G, SIGMA, RHS>(B, j0 + G, c);
-          fwd_run<W, G, SIGMA, RHS>(A, j0, c, xbuf, XS, xsel, zprev);
-          fwd_load<W, G, SIGMA, RHS>(A, j0 + 2 * G, c);
-          fwd_run<W, G, SIGMA, RHS>(B, j0 + G, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, PB, SIGMA, RHS>(B, j0 + G, c);
+          fwd_run<W, G, PB, SIGMA, RHS>(A, j0, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, PB, SIGMA, RHS>(A, j0 + 2 * G, c);
+          fwd_run<W, G, PB, SIGMA, RHS>(B, j0 + G, c, xbuf, XS, xsel, zprev);
         }
       } else {
-        fwd_load<W, G, SIGMA, false>(A, 0, c);
+        fwd_load<W, G, PB, SIGMA, false>(A, 0, c);
         for (int j0 = 0; j0 < ny; j0 += 2 * G) {
-          fwd_load<W, G, SIGMA, false>(B, j0 + G, c);
-          fwd_run<W, G, SIGMA, false>(A, j0, c, xbuf, XS, xsel, zprev);
-          fwd_load<W, G, SIGMA, false>(A, j0 + 2 * G, c);
-          fwd_run<W, G, SIGMA, false>(B, j0 + G, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, PB, SIGMA, false>(B, j0 + G, c);
+          fwd_run<W, G, PB, SIGMA, false>(A, j0, c, xbuf, XS, xsel, zprev);
+          fwd_load<W, G, PB, SIGMA, false>(A, j0 + 2 * G, c);
+          fwd_run<W, G, PB, SIGMA, false>(B, j0 + G, c, xbuf, XS, xsel, zprev);
         }
       }
     }
 
     // back substitution: x_j = z_j - e_j W_j x_{j+1}; u_s = 2x - u_{s-1} (sigma: x - u_{s-1})
     {
-      float2 xnext = zero;
-      BwdBank<W, G> A, B;
-      bwd_load<W, G>(A, ny - 1, c);
+      float2 xnext[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) xnext[k] = zero;
+      BwdBank<W, G, PB> A, B;
+      bwd_load<W, G, PB>(A, ny - 1, c);
       for (int j0 = ny - 1; j0 >= 0; j0 -= 2 * G) {
-        bwd_load<W, G>(B, j0 - G, c);
-        bwd_run<W, G, SIGMA>(A, j0, c, xbuf, XS, xsel, xnext);
-        bwd_load<W, G>(A, j0 - 2 * G, c);
-        bwd_run<W, G, SIGMA>(B, j0 - G, c, xbuf, XS, xsel, xnext);
+        bwd_load<W, G, PB>(B, j0 - G, c);
+        bwd_run<W, G, PB, SIGMA>(A, j0, c, xbuf, XS, xsel, xnext);
+        bwd_load<W, G, PB>(A, j0 - 2 * G, c);
+        bwd_run<W, G, PB, SIGMA>(B, j0 - G, c, xbuf, XS, xsel, xnext);
       }
     }
 
     if (act) {
-      if (d.exitp && s == L) copy_rows((float2*)d.exitp + (size_t)p * d.exit_pstride, c.cur, nx, ny, i);
-      if (d.ckpt && (s % d.ckpt_every) == 0)
-        copy_rows((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, c.cur,
-                  nx, ny, i);
+      for (int k = 0; k < np; ++k) {
+        const float2* src = c.cur + k * c.cur_ps;
+        if (d.exitp && s == L) copy_rows((float2*)d.exitp + (size_t)(p0 + k) * d.exit_pstride, src, nx, ny, i);
+        if (d.ckpt && (s % d.ckpt_every) == 0)
+          copy_rows((float2*)d.ckpt + (size_t)(p0 + k) * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride,
+                    src, nx, ny, i);
+      }
     }
   }
 }
 
-template <int W, bool SIGMA>
+template <int W, int PB, bool SIGMA>
 static int launch_band_sweep(const BandSweepArgs& a, int nt, bool rhs, hipStream_t st) {
-  const size_t lds = (size_t)2 * (nt + 2 * W) * sizeof(float2);
+  const size_t lds = (size_t)2 * PB * (nt + 2 * W) * sizeof(float2);
   if (rhs)
-    hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA, true>), dim3(a.nblk), dim3(nt), lds, st, a);
+    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, true>), dim3(a.nblk), dim3(nt), lds, st, a);
   else
-    hipLaunchKernelGGL((band_sweep_kernel<W, SIGMA, false>), dim3(a.nblk), dim3(nt), lds, st, a);
+    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, false>), dim3(a.nblk), dim3(nt), lds, st, a);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
 
+template <int W, bool SIGMA>
+static int launch_band_sweep_pb(const BandSweepArgs& a, int nt, bool rhs, int pb, hipStream_t st) {
+  switch (pb) {
+    case 1:
+      return launch_band_sweep<W, 1, SIGMA>(a, nt, rhs, st);
+    case 2:
+      return launch_band_sweep<W, 2, SIGMA>(a, nt, rhs, st);
+    case 4:
+      return launch_band_sweep<W, 4, SIGMA>(a, nt, rhs, st);
+  }
+  set_error("tp_band_sweep: probes_per_block must be 1, 2 or 4 (got %d)", pb);
+  return TP_ERR_INVALID;
+}
+
 static bool band_w_supported(int w) {
   return w == 4 || w == 6 || w == 8 || w == 12 || w == 16 || w == 24 || w == 32;
+}
+
+// ------------------------------------------------------------------ factors of A^H from those of A
+// A^H has blocks A_jj^H and couplings conj(e_{j-1}) (below) and conj(f_{j+1})
+// (above), so its block recurrence gives S'_j = (A_jj - f_j e_{j-1} W_{j-1})^H =
+// S_j^H and W'_j = W_j^H: W'_j(i, i+d-w) = conj(W_j(i+d-w, i)), which is band
+// index 2w - d of row i + d - w.  With `reverse`, slot a of the output holds
+// slice steps-1-a of the input (the adjoint sweep walks the slices backwards).
+__global__ __launch_bounds__(256) void band_conj_transpose_kernel(const float2* __restrict__ src,
+                                                                  float2* __restrict__ dst, int nx, int ny, int steps,
+                                                                  int w, int reverse) {
+  const int BW = 2 * w + 1;
+  const int PW = (BW + 1) * nx;  // complex values per block in the pair layout
+  const int blk = blockIdx.x;    // slot * ny + j
+  const int slot = blk / ny, j = blk - slot * ny;
+  const int sslot = reverse ? steps - 1 - slot : slot;
+  const float2* S = src + ((size_t)sslot * ny + j) * PW;
+  float2* D = dst + (size_t)blk * PW;
+  for (int t = threadIdx.x; t < PW; t += blockDim.x) {
+    const int pr = t >> 1, dp = pr / nx, i = pr - dp * nx, d = 2 * dp + (t & 1);
+    const int r = i + d - w;
+    float2 v = make_float2(0.f, 0.f);
+    if (d < BW && r >= 0 && r < nx) {
+      const int dd = 2 * w - d;
+      const float2 x = S[((dd >> 1) * nx + r) * 2 + (dd & 1)];
+      v = make_float2(x.x, -x.y);
+    }
+    D[t] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void band_sigma_kernel(const double2* C, int64_t N, int ncol, double lsign,
+                                                         double rsign, int conj_c, int lcol0, int rcol0, int cstep,
+                                                         float2* sigma) {
+  const int s = blockIdx.y;
+  const int lc = lcol0 + cstep * s, rc = rcol0 + cstep * s;
+  const bool lok = lc >= 0 && lc < ncol, rok = rc >= 0 && rc < ncol;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x)
+    sigma[(size_t)s * N + t] = band_sigma_at(C, ncol, t, lc, lok, rc, rok, lsign, rsign, conj_c);
 }
 
 }  // namespace tp
@@ -559,9 +666,45 @@ extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const v
   return TP_OK;
 }
 
-extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream) {
+extern "C" int tp_band_conj_transpose(const void* d_W, int nx, int ny, int steps, int w, int reverse_slices,
+                                      void* d_WH, void* stream) {
+  if (!d_W || !d_WH || d_W == d_WH) {
+    set_error("tp_band_conj_transpose: need distinct input and output buffers");
+    return TP_ERR_INVALID;
+  }
+  if (nx < 2 || ny < 2 || steps < 1 || !band_w_supported(w) || w >= nx) {
+    set_error("tp_band_conj_transpose: invalid sizes (nx=%d, ny=%d, steps=%d, w=%d)", nx, ny, steps, w);
+    return TP_ERR_INVALID;
+  }
+  hipLaunchKernelGGL(band_conj_transpose_kernel, dim3((unsigned)(steps * ny)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)d_W, (float2*)d_WH, nx, ny, steps, w, reverse_slices);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_band_sigma(const void* d_C, int nx, int ny, int ncol, double lhs_sign, double rhs_sign, int conj_c,
+                             int lhs_col0, int rhs_col0, int col_step, int steps, void* d_sigma, void* stream) {
+  if (!d_C || !d_sigma || nx < 1 || ny < 1 || ncol < 1 || steps < 1) {
+    set_error("tp_band_sigma: invalid arguments (nx=%d, ny=%d, ncol=%d, steps=%d)", nx, ny, ncol, steps);
+    return TP_ERR_INVALID;
+  }
+  const int64_t N = (int64_t)nx * ny;
+  const unsigned gx = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
+  hipLaunchKernelGGL(band_sigma_kernel, dim3(gx, (unsigned)steps), dim3(256), 0, (hipStream_t)stream,
+                     (const double2*)d_C, N, ncol, lhs_sign, rhs_sign, conj_c, lhs_col0, rhs_col0, col_step,
+                     (float2*)d_sigma);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_per_block, void* stream) {
   if (!scans || nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("tp_band_sweep: nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
+    return TP_ERR_INVALID;
+  }
+  const int pb = probes_per_block;
+  if (pb != 1 && pb != 2 && pb != 4) {
+    set_error("tp_band_sweep: probes_per_block must be 1, 2 or 4 (got %d)", pb);
     return TP_ERR_INVALID;
   }
   BandSweepArgs a;
@@ -598,7 +741,7 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream
     rhs = rhs || d.rhs != nullptr;
     a.scans[k] = d;
     a.blk_start[k] = nblk;
-    nblk += d.probes;
+    nblk += (d.probes + pb - 1) / pb;
   }
   a.blk_start[nscans] = nblk;
   a.nscans = nscans;
@@ -607,7 +750,7 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, void* stream
   hipStream_t st = (hipStream_t)stream;
 #define TP_BAND_CASE(WV)                                                                           \
   case WV:                                                                                        \
-    return sigma ? launch_band_sweep<WV, true>(a, nt, rhs, st) : launch_band_sweep<WV, false>(a, nt, rhs, st);
+    return sigma ? launch_band_sweep_pb<WV, true>(a, nt, rhs, pb, st) : launch_band_sweep_pb<WV, false>(a, nt, rhs, pb, st);
   switch (w) {
     TP_BAND_CASE(4)
     TP_BAND_CASE(6)

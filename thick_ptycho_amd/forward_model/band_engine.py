@@ -243,10 +243,18 @@ class BandFactorizer:
     captured into a HIP graph); ``check()`` synchronises and verifies the band edge and
     the pivots of the last run."""
 
-    def __init__(self, A: Block3D, B: Block3D, mode: str, L: int, w: int, device):
+    def __init__(self, A: Block3D, B: Block3D, mode: str, L: int, w: int, device,
+                 source: Optional["BandFactorizer"] = None):
         nx, ny = A.nx, A.ny
         self.plan = band_mode_plan(A, B, mode, L)
         self.plan.check_complementary()
+        # adjoint modes from the forward factors of the same object: their LHS of
+        # step s is the conjugate transpose of the forward LHS of slice L - s + 1
+        if source is not None:
+            assert mode in ("adjoint", "exact_adjoint") and source.plan is not None
+            assert source.mode == "forward" and (source.nx, source.ny, source.L, source.w) == (nx, ny, L, w)
+        self.source = source
+        self.mode = mode
         self.nx, self.ny, self.L, self.w = nx, ny, L, w
         self.tri = torch.from_numpy(np.ascontiguousarray(self.plan.lhs.tri)).to(device)
         self.cpl = torch.from_numpy(np.ascontiguousarray(np.stack([self.plan.lhs.f, self.plan.lhs.e], axis=1))).to(device)
@@ -261,6 +269,15 @@ class BandFactorizer:
 
     def run(self, C: torch.Tensor) -> BandFactors:
         pl = self.plan
+        if self.source is not None:  # the source must have run on the same C
+            lib = _lib.load()
+            _lib.check(lib.tp_band_conj_transpose(_lib.ptr(self.source.fac.W), self.nx, self.ny, self.L, self.w, 1,
+                                                  _lib.ptr(self.fac.W), _lib.stream_handle()), "tp_band_conj_transpose")
+            if pl.sigma:
+                _lib.check(lib.tp_band_sigma(_lib.ptr(C), self.nx, self.ny, self.L, pl.lhs_sign, pl.rhs_sign, pl.conj_c,
+                                             pl.lhs_col0, pl.rhs_col0, pl.col_step, self.L, _lib.ptr(self.fac.sigma),
+                                             _lib.stream_handle()), "tp_band_sigma")
+            return self.fac
         _lib.check(
             _lib.load().tp_band_factor(_lib.ptr(C), self.nx, self.ny, self.L, _lib.ptr(self.tri), _lib.ptr(self.cpl),
                                        pl.lhs_sign, pl.rhs_sign, pl.conj_c, pl.lhs_col0, pl.rhs_col0, pl.col_step,
@@ -271,6 +288,9 @@ class BandFactorizer:
         return self.fac
 
     def check(self) -> float:
+        if self.source is not None:
+            self.fac.edge = self.source.check()
+            return self.fac.edge
         if int(self.status.item()) != 0:
             raise RuntimeError("singular per-slice factor (zero pivot) in tp_band_factor")
         e = float(self.edge.max().item())
@@ -342,8 +362,11 @@ class BandScanSpec:
         return s
 
 
-def sweep(specs: Sequence[BandScanSpec], stream=None) -> None:
-    """All probes of up to eight batches through every slice in one launch."""
+def sweep(specs: Sequence[BandScanSpec], probes_per_block: int = 1, stream=None) -> None:
+    """All probes of up to eight batches through every slice in one launch;
+    ``probes_per_block`` (1, 2, 4) probes share each workgroup and each load of
+    the band coefficients."""
     assert 1 <= len(specs) <= 8
     arr = (_lib.BandScan * len(specs))(*[sp.struct() for sp in specs])
-    _lib.check(_lib.load().tp_band_sweep(arr, len(specs), _lib.stream_handle(stream)), "tp_band_sweep")
+    _lib.check(_lib.load().tp_band_sweep(arr, len(specs), int(probes_per_block), _lib.stream_handle(stream)),
+               "tp_band_sweep")

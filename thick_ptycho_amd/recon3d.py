@@ -28,13 +28,14 @@ from .forward_model import band_engine as be
 
 class Reconstruction3D:
     def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
-                 batch: Optional[int] = None, w: Optional[int] = None, device=None):
+                 batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None):
         dev = device or torch.device("cuda", torch.cuda.current_device())
         self.dev = dev
         n0 = np.asarray(n0)
         nx, ny, nz = n0.shape
         self.nx, self.ny, self.nz, self.N, self.L = nx, ny, nz, nx * ny, nz - 1
         self.k, self.dz = k, dz
+        self.ppb = probes_per_block
         P = probes.shape[0]
         self.P = P
         self.batch = batch or P
@@ -47,7 +48,7 @@ class Reconstruction3D:
         w = w or be.factor(self.C, A, B, "forward").w  # band half-width chosen once, at n0
         self.w = w
         self.fwd = be.BandFactorizer(A, B, "forward", self.L, w, dev)
-        self.adj = be.BandFactorizer(A, B, "exact_adjoint", self.L, w, dev)
+        self.adj = be.BandFactorizer(A, B, "exact_adjoint", self.L, w, dev, source=self.fwd)
         Pb = self.batch
         self.u = torch.empty((Pb, self.L + 1, self.N), dtype=torch.complex64, device=dev)
         self.lam = torch.empty_like(self.u)
@@ -80,14 +81,14 @@ class Reconstruction3D:
             b1 = min(b0 + self.batch, self.P)
             nb = b1 - b0
             u, lam = self.u[:nb], self.lam[:nb]
-            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)])
+            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb)
             ex = self.exit[:nb]
             ex.copy_(u[:, self.L])
             _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb, 1.0,
                                                _lib.ptr(self.seed[:nb]), _lib.ptr(self.loss[b0:b1]),
                                                _lib.ptr(self.work), sh()), "tp_amplitude_seed2d")
             rhs = self.seed[:nb] if seed is None else seed[b0:b1].reshape(nb, 1, self.N)
-            be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)])
+            be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)], self.ppb)
             _lib.check(lib.tp_grad3d_accumulate(_lib.ptr(u), _lib.ptr(lam), nb, self.L, self.N, _lib.ptr(self.G), sh()),
                        "tp_grad3d_accumulate")
         Gt = self.G.transpose(0, 1).to(torch.complex128)
@@ -113,6 +114,23 @@ class Reconstruction3D:
         Lc = float(torch.linalg.vector_norm(self.grad - g0) / torch.linalg.vector_norm(d))
         self.n.copy_(n0)
         return Lc
+
+    def tune(self, candidates=(1, 2, 4)) -> dict:
+        """Pick the probes per workgroup of the band sweeps by timing one gradient
+        evaluation with each; returns {ppb: ms}."""
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        times = {}
+        for ppb in candidates:
+            self.ppb = ppb
+            self.gradient()
+            e0.record()
+            self.gradient()
+            e1.record()
+            torch.cuda.synchronize()
+            times[ppb] = e0.elapsed_time(e1)
+        self.ppb = min(times, key=times.get)
+        self.graph = None
+        return times
 
     def capture(self) -> None:
         """Capture one iteration (gradient + update) into a HIP graph."""
