@@ -79,6 +79,11 @@ __device__ __forceinline__ float2 twid(const float2* tw, int e) {
   return SIGN > 0 ? cconj(w) : w;
 }
 
+// LDS position of element i of a Stockham buffer: one padding slot after every
+// 16 complex values (128 bytes), so that the strided writes of the register
+// stage (lane*R + r) and of the merge stages fall into distinct banks.
+__device__ __forceinline__ constexpr int lp(int i) { return i + (i >> 4); }
+
 // Stockham stage (Govindaraju et al., SC'08 formulation): sub-transforms of
 // size NS are merged by radix RS.  Butterfly j reads src[j + r*N/RS] and writes
 // dst[(j/NS)*NS*RS + j%NS + r*NS].
@@ -92,7 +97,7 @@ __device__ __forceinline__ void stage_lds(const float2* src, float2* dst, const 
     const int j = lane + 64 * t;
     float2 v[RS];
 #pragma unroll
-    for (int r = 0; r < RS; ++r) v[r] = src[j + r * B];
+    for (int r = 0; r < RS; ++r) v[r] = src[lp(j + r * B)];
     const int jm = j & (NS - 1);
     if (NS > 1) {
 #pragma unroll
@@ -101,7 +106,7 @@ __device__ __forceinline__ void stage_lds(const float2* src, float2* dst, const 
     dft_reg<RS, SIGN>(v);
     const int base = (j / NS) * NS * RS + jm;
 #pragma unroll
-    for (int r = 0; r < RS; ++r) dst[base + r * NS] = v[r];
+    for (int r = 0; r < RS; ++r) dst[lp(base + r * NS)] = v[r];
   }
 }
 
@@ -117,7 +122,7 @@ __device__ __forceinline__ void stage_to_reg(const float2* src, float2* x, const
     const int j = lane + 64 * t;
     float2 v[RS];
 #pragma unroll
-    for (int r = 0; r < RS; ++r) v[r] = src[j + r * B];
+    for (int r = 0; r < RS; ++r) v[r] = src[lp(j + r * B)];
 #pragma unroll
     for (int r = 1; r < RS; ++r) v[r] = cmul(v[r], twid<SIGN>(tw, j * r));
     dft_reg<RS, SIGN>(v);
@@ -132,11 +137,11 @@ template <int R, int SIGN>
 __device__ __forceinline__ void stage_from_reg(float2* x, float2* dst, int lane) {
   dft_reg<R, SIGN>(x);
 #pragma unroll
-  for (int r = 0; r < R; ++r) dst[lane * R + r] = x[r];
+  for (int r = 0; r < R; ++r) dst[lp(lane * R + r)] = x[r];
 }
 
 // Full FFT of n = 64 R points, register layout in and out; buffers a, b are
-// per-wavefront LDS scratch of n points each.
+// per-wavefront LDS scratch of ms_bufn<R>() points each (n plus padding).
 template <int R, int SIGN>
 __device__ __forceinline__ void fft_wave(float2* x, float2* a, float2* b, const float2* tw, int lane) {
   constexpr int N = 64 * R;
@@ -266,7 +271,7 @@ __device__ __forceinline__ void fft_any(float2* x, float2* a, float2* b, const f
 // LDS ping-pong buffers are needed only by the power-of-two Stockham path
 template <int R>
 constexpr int ms_bufn() {
-  return ms_pow2<R>() ? 64 * R : 1;
+  return ms_pow2<R>() ? 64 * R + 4 * R : 1;  // lp(64 R - 1) + 1
 }
 
 

@@ -54,17 +54,19 @@ __device__ __forceinline__ void ms_slice(float2* x, const float2* T, const float
   fft_any<R, +1>(x, a, b, tw, lane);
 }
 
-template <int R>
-__global__ __launch_bounds__(64 * MS_PPB) void ms_sweep_kernel(const MsArgs a) {
+// PPB probes (one per wavefront) per workgroup: 4 when the launch has enough
+// probes to give every CU a workgroup, else 1 so that the waves spread over all CUs.
+template <int R, int PPB>
+__global__ __launch_bounds__(64 * PPB) void ms_sweep_kernel(const MsArgs a) {
   constexpr int N = 64 * R;
   __shared__ float2 tw[N];
-  __shared__ float2 buf[MS_PPB][2][ms_bufn<R>()];
+  __shared__ float2 buf[PPB][2][ms_bufn<R>()];
   const int blk = blockIdx.x;
   const int sc = ms_find_scan(a, blk);
   const tp_ms_scan& d = a.scans[sc];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int p = (blk - a.blk_start[sc]) * MS_PPB + wave;
+  const int p = (blk - a.blk_start[sc]) * PPB + wave;
   build_twiddles(tw, N);
   __syncthreads();
   if (p >= d.probes) return;
@@ -222,7 +224,7 @@ static int ms_radix(int n, int& r) {
   X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
   X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
 
-static int ms_build(const tp_ms_scan* scans, int nscans, MsArgs& a, int& nblk, int& r) {
+static int ms_build(const tp_ms_scan* scans, int nscans, MsArgs& a, int& nblk, int& r, int ppb = MS_PPB) {
   if (nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
     return TP_ERR_INVALID;
@@ -246,7 +248,7 @@ static int ms_build(const tp_ms_scan* scans, int nscans, MsArgs& a, int& nblk, i
     }
     a.scans[i] = d;
     a.blk_start[i] = nblk;
-    nblk += (d.probes + MS_PPB - 1) / MS_PPB;
+    nblk += (d.probes + ppb - 1) / ppb;
   }
   a.blk_start[nscans] = nblk;
   a.nscans = nscans;
@@ -287,10 +289,20 @@ extern "C" int tp_ms_sweep(const tp_ms_scan* scans, int nscans, void* stream) {
   int rc = ms_build(scans, nscans, a, nblk, r);
   if (rc) return rc;
   if (nblk == 0) return TP_OK;
+  const bool spread = nblk < 256;  // fewer 4-probe workgroups than CUs
+  if (spread) {
+    rc = ms_build(scans, nscans, a, nblk, r, 1);
+    if (rc) return rc;
+  }
   hipStream_t st = (hipStream_t)stream;
   switch (r) {
-#define TP_X(RV) \
-  case RV: hipLaunchKernelGGL((ms_sweep_kernel<RV>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a); break;
+#define TP_X(RV)                                                                                         \
+  case RV:                                                                                              \
+    if (spread)                                                                                         \
+      hipLaunchKernelGGL((ms_sweep_kernel<RV, 1>), dim3(nblk), dim3(64), 0, st, a);                     \
+    else                                                                                                \
+      hipLaunchKernelGGL((ms_sweep_kernel<RV, MS_PPB>), dim3(nblk), dim3(64 * MS_PPB), 0, st, a);       \
+    break;
     TP_MS_R_CASES(TP_X)
 #undef TP_X
   }
