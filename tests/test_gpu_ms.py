@@ -33,7 +33,9 @@ def test_ms_matches_oracle(gpu, nx, rng):
     assert rel_l2(one, ref[2]) < TOL
     plain = s.solve_device(n=n, plain=True)
     pers = s.solve_device(n=n)
-    assert rel_l2(plain.cpu().numpy(), pers.cpu().numpy()) < 1e-6
+    # n = 512, 1024: the persistent form splits each FFT over 2 or 4 wavefronts
+    # (another decomposition, so float32 rounding differs from the plain form)
+    assert rel_l2(plain.cpu().numpy(), pers.cpu().numpy()) < 1e-5
 
 
 def test_ms_plane_wave_eigen(gpu):

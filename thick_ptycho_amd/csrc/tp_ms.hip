@@ -94,6 +94,106 @@ __global__ __launch_bounds__(64 * PPB) void ms_sweep_kernel(const MsArgs a) {
   }
 }
 
+// Split form (n = 64 R, R in {8, 16}): S wavefronts share one probe, so that a
+// launch of P probes keeps S P wavefronts in flight (the one-wavefront form
+// leaves SIMDs idle and is bound by the latency of its LDS stages).  With
+// M = n / S, n = n2 + M n1 and k = k1 + S k2,
+//   X[k1 + S k2] = sum_n2 W_M^{n2 k2} [ W_n^{n2 k1} sum_n1 W_S^{n1 k1} x[n2 + M n1] ],
+// so wavefront w keeps x[w M + n2] (contiguous loads and stores), an S-point
+// DFT across the wavefronts (through LDS) turns it into the k1 = w
+// sub-sequence, and a wave-level M-point FFT finishes the transform, leaving
+// X[w + S k2] at lane/register k2 = l + 64 m; the inverse runs the steps
+// backwards with conjugate twiddles.
+template <int S, int RM, int SIGN>
+__device__ __forceinline__ void ms_cross(float2* x, float2* ta, float2* tb, const float2* twN, int w, int lane) {
+  constexpr int M = 64 * RM, N = S * M, QL = M / S / 64;
+  if (SIGN > 0) {  // inverse: twiddle W_n^{+n2 k1} (k1 = w) before the S-point DFT
+#pragma unroll
+    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], twid<+1>(twN, ((lane + 64 * m) * w) & (N - 1)));
+  }
+#pragma unroll
+  for (int m = 0; m < RM; ++m) ta[lp(w * M + lane + 64 * m)] = x[m];
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < QL; ++t) {
+    const int n2 = w * (M / S) + lane + 64 * t;
+    float2 v[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) v[r] = ta[lp(r * M + n2)];
+    dft_reg<S, SIGN>(v);
+    if (SIGN < 0) {  // forward: twiddle W_n^{n2 k1} after the S-point DFT
+#pragma unroll
+      for (int k = 1; k < S; ++k) v[k] = cmul(v[k], twid<-1>(twN, (n2 * k) & (N - 1)));
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) tb[lp(k * M + n2)] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < RM; ++m) x[m] = tb[lp(w * M + lane + 64 * m)];
+}
+
+template <int R, int S>
+__global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
+  constexpr int N = 64 * R, M = N / S, RM = R / S;
+  static_assert(ms_pow2<RM>() && M % (64 * S) == 0, "split form needs a power-of-two wave FFT");
+  __shared__ float2 twN[N];
+  __shared__ float2 twM[M];
+  __shared__ float2 ta[N + N / 16], tb[N + N / 16];
+  __shared__ float2 buf[S][2][ms_bufn<RM>()];
+  const int blk = blockIdx.x;  // one probe per workgroup
+  const int sc = ms_find_scan(a, blk);
+  const tp_ms_scan& d = a.scans[sc];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int p = blk - a.blk_start[sc];
+  build_twiddles(twN, N);
+  build_twiddles(twM, M);
+  __syncthreads();
+  float2 H[RM], T[RM], Tn[RM], x[RM];
+  const float2* Hg = (const float2*)d.H;
+  const float2* Tg = (const float2*)d.T + w * M;
+  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride + w * M;
+#pragma unroll
+  for (int m = 0; m < RM; ++m) {
+    H[m] = Hg[w + S * (lane + 64 * m)];
+    if (d.conj_h) H[m] = cconj(H[m]);
+    x[m] = u0[lane + 64 * m];
+    T[m] = Tg[lane + 64 * m];
+  }
+  float2* outp = d.out ? (float2*)d.out + (size_t)p * d.out_pstride + w * M : nullptr;
+  if (outp) {
+#pragma unroll
+    for (int m = 0; m < RM; ++m) outp[lane + 64 * m] = x[m];
+  }
+  for (int z = 0; z < d.steps; ++z) {
+    if (z + 1 < d.steps) {
+#pragma unroll
+      for (int m = 0; m < RM; ++m) Tn[m] = Tg[(size_t)(z + 1) * N + lane + 64 * m];
+    }
+#pragma unroll
+    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], T[m]);
+    ms_cross<S, RM, -1>(x, ta, tb, twN, w, lane);
+    fft_wave<RM, -1>(x, buf[w][0], buf[w][1], twM, lane);
+#pragma unroll
+    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], H[m]);
+    fft_wave<RM, +1>(x, buf[w][0], buf[w][1], twM, lane);
+    ms_cross<S, RM, +1>(x, ta, tb, twN, w, lane);
+    if (outp) {
+      float2* o = outp + (int64_t)(z + 1) * d.out_zstride;
+#pragma unroll
+      for (int m = 0; m < RM; ++m) o[lane + 64 * m] = x[m];
+    }
+#pragma unroll
+    for (int m = 0; m < RM; ++m) T[m] = Tn[m];
+  }
+  if (d.exitp) {
+    float2* o = (float2*)d.exitp + (size_t)p * d.exit_pstride + w * M;
+#pragma unroll
+    for (int m = 0; m < RM; ++m) o[lane + 64 * m] = x[m];
+  }
+}
+
 template <int R>
 __global__ __launch_bounds__(64 * MS_PPB) void ms_step_kernel(const MsArgs a, int z) {
   constexpr int N = 64 * R;
@@ -289,12 +389,22 @@ extern "C" int tp_ms_sweep(const tp_ms_scan* scans, int nscans, void* stream) {
   int rc = ms_build(scans, nscans, a, nblk, r);
   if (rc) return rc;
   if (nblk == 0) return TP_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (r == 8 || r == 16) {  // split form: one probe per workgroup of S wavefronts
+    rc = ms_build(scans, nscans, a, nblk, r, 1);
+    if (rc) return rc;
+    if (r == 8)
+      hipLaunchKernelGGL((ms_split_kernel<8, 2>), dim3(nblk), dim3(128), 0, st, a);
+    else
+      hipLaunchKernelGGL((ms_split_kernel<16, 4>), dim3(nblk), dim3(256), 0, st, a);
+    TP_CHECK_HIP(hipGetLastError());
+    return TP_OK;
+  }
   const bool spread = nblk < 256;  // fewer 4-probe workgroups than CUs
   if (spread) {
     rc = ms_build(scans, nscans, a, nblk, r, 1);
     if (rc) return rc;
   }
-  hipStream_t st = (hipStream_t)stream;
   switch (r) {
 #define TP_X(RV)                                                                                         \
   case RV:                                                                                              \
