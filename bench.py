@@ -212,7 +212,16 @@ def main() -> int:
         extras["multislice_plain_value"] = sc.steps / t_ms_plain
         log(f"multislice: {extras['multislice_value']:.4g} steps/s persistent, "
             f"{extras['multislice_plain_value']:.4g} plain")
+        # multislice amplitude gradient on the same scan: forward + conj(H) adjoint
+        # sweeps with every plane kept, loss seed and per-slice accumulation
+        from thick_ptycho_amd import gradient as tg
+
+        sqrt_i = torch.sqrt(ms_engine.farfield(ms_out[:, -1].contiguous())) * 1.05
         del ms_out
+        t_g = timed(lambda: tg.ms_amplitude_gradient(n_dev, main["u0"], sqrt_i, sc.dx, sc.k, sc.dz), 2, 1)
+        extras["multislice_gradient_value"] = 2 * sc.steps / t_g
+        extras["multislice_gradient_ms"] = t_g * 1e3
+        log(f"multislice gradient: {t_g * 1e3:.2f} ms, {extras['multislice_gradient_value']:.4g} steps/s (fwd+adj)")
 
     if not args.no_3d:
         extras["config3"] = bench_3d(dev, args.steps)

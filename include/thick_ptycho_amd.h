@@ -303,6 +303,15 @@ int tp_farfield2d(const void* d_exit, int nx, int ny, int probes, void* d_intens
  * a: float32 [probes][nx][ny]; d_work: 2 * probes * nx * ny complex64. */
 int tp_amplitude_seed2d(const void* d_exit, const void* d_sqrt_i, int nx, int ny, int probes, double scale,
                         void* d_seed, void* d_loss, void* d_work, void* stream);
+/* Per-slice plane products summed over probes (the multislice gradient's
+ * accumulation, the adjoint of ms_solver.py:143-158 with respect to the
+ * transmission; no counterpart in the reference, which has no MS gradient):
+ * G[z][f] += sum_p conj(a[p*a_pstride + (a_z0 + a_zstep z) N + f]) *
+ *                    b[p*b_pstride + (b_z0 + b_zstep z) N + f],  z < nplanes. */
+int tp_plane_conj_dot(const void* d_a, int64_t a_pstride, int a_z0, int a_zstep, const void* d_b,
+                      int64_t b_pstride, int b_z0, int b_zstep, int probes, int nplanes, int64_t N,
+                      void* d_G, void* stream);
+
 /* G[s-1][f] += sum_p conj(lam_p[L-s+1][f]) (u_p[s][f] + u_p[s-1][f]) for s = 1..L (= steps):
  * the per-slice gradient accumulation of two transverse axes from full fields,
  * u (forward) and lam (exact adjoint in sweep order), both complex64

@@ -137,3 +137,58 @@ def solve_batched_2d(n: np.ndarray, probes: np.ndarray, dx: float, dy: float, k:
         if out is not None:
             out[..., z + 1] = psi
     return out if out is not None else psi
+
+
+# ----------------------------------------------------------------------------
+# Exact gradient of the far-field amplitude misfit through the multislice
+# (the differentiable form of ms_solver.py:143-158; the reference's MS
+# reconstruction, ms_reconstructor.py:109-190, is a 3PIE update instead).
+# psi_{z+1} = P (T_z psi_z) with P = F^-1 H F, so with lambda_L = dJ/d conj(psi_L):
+#   mu_z = P^H lambda_{z+1} = F^-1 conj(H) F lambda_{z+1},  lambda_z = conj(T_z) mu_z,
+#   dJ/dRe n_z + i dJ/dIm n_z = 2 conj(dT_z/dn_z) sum_p conj(psi_z) mu_z
+#                             = -2 i k dz sum_p conj(T_z psi_z) mu_z,
+# and the last column of n (never used by the loop) has zero gradient.
+# ----------------------------------------------------------------------------
+
+
+def _ms_gradient(n, psi0, k, dz, n_medium, H, fwd, inv, loss_fn, seed_fn, seed):
+    nz = n.shape[-1]
+    T = transmission(n, k, dz, n_medium)
+    psi = [np.asarray(psi0, dtype=complex)]
+    for z in range(nz - 1):
+        psi.append(inv(fwd(psi[-1] * T[..., z]) * H))
+    J = loss_fn(psi[-1])
+    lam = seed_fn(psi[-1]) if seed is None else np.asarray(seed, dtype=complex).reshape(psi[-1].shape)
+    grad = np.zeros(n.shape, dtype=complex)
+    for z in range(nz - 2, -1, -1):
+        mu = inv(fwd(lam) * np.conj(H))
+        grad[..., z] = -2j * k * dz * np.sum(np.conj(T[..., z] * psi[z]) * mu, axis=0)
+        lam = np.conj(T[..., z]) * mu
+    return grad, J
+
+
+def ms_gradient(n: np.ndarray, probes: np.ndarray, dx: float, k: float, dz: float, I_meas: np.ndarray,
+                n_medium: float = 1.0, seed: Optional[np.ndarray] = None):
+    """(dJ/dRe n + i dJ/dIm n of shape (nx, nz), J) for J = sum (|fft(psi_L)| - sqrt(I))^2,
+    one transverse axis; ``seed`` (P, nx) replaces dJ/d conj(psi_L) when given."""
+    from .gradient import amplitude_loss, amplitude_loss_seed
+
+    nx = n.shape[0]
+    H = propagation_kernel(nx, dx, k, dz, True)
+    return _ms_gradient(n, np.asarray(probes).reshape(-1, nx), k, dz, n_medium, H,
+                        lambda a: np.fft.fft(a, axis=-1), lambda a: np.fft.ifft(a, axis=-1),
+                        lambda e: amplitude_loss(e, I_meas), lambda e: amplitude_loss_seed(e, I_meas), seed)
+
+
+def ms_gradient_2d(n: np.ndarray, probes: np.ndarray, dx: float, dy: float, k: float, dz: float,
+                   I_meas: np.ndarray, n_medium: float = 1.0, seed: Optional[np.ndarray] = None):
+    """Two-axis form: n (nx, ny, nz), probes (P, nx, ny), the per-probe fft2 far field."""
+    from .gradient import amplitude_loss_2d, amplitude_loss_seed_2d
+
+    nx, ny, _ = n.shape
+    P = probes.shape[0]
+    H = propagation_kernel_2d(nx, ny, dx, dy, k, dz)
+    return _ms_gradient(n, np.asarray(probes).reshape(P, nx, ny), k, dz, n_medium, H,
+                        lambda a: np.fft.fft2(a, axes=(-2, -1)), lambda a: np.fft.ifft2(a, axes=(-2, -1)),
+                        lambda e: amplitude_loss_2d(e, I_meas, nx, ny),
+                        lambda e: amplitude_loss_seed_2d(e, I_meas, nx, ny).reshape(P, nx, ny), seed)

@@ -1,6 +1,7 @@
 """Pin the FP64 multislice / far-field restatement with closed forms (CPU)."""
 
 import numpy as np
+import pytest
 
 from oracle import multislice as oms
 from tests.cases import rel_l2
@@ -98,3 +99,39 @@ def test_multislice_2d_unitary_in_free_space():
     probes = rng.standard_normal((2, nx, ny)) + 1j * rng.standard_normal((2, nx, ny))
     u = solve_batched_2d(np.ones((nx, ny, nz)), probes, 2.0, 2.0, 2 * np.pi, 4.0, keep="exit")
     assert np.allclose(np.linalg.norm(u.reshape(2, -1), axis=1), np.linalg.norm(probes.reshape(2, -1), axis=1))
+
+
+@pytest.mark.parametrize("axes", [1, 2])
+def test_ms_gradient_finite_differences(axes):
+    """The multislice amplitude-loss gradient against central differences of the
+    loss in FP64 (real and imaginary parts of n at seeded entries)."""
+    rng = np.random.default_rng(5)
+    k, dz = 2 * np.pi, 4.0
+    if axes == 1:
+        nx, nz, P = 48, 9, 3
+        shape = (nx, nz)
+        probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+        I = np.abs(rng.standard_normal((P, nx))) ** 2 * nx
+
+        def run(n):
+            return oms.ms_gradient(n, probes, 2.0, k, dz, I)
+    else:
+        nx, ny, nz, P = 16, 12, 7, 2
+        shape = (nx, ny, nz)
+        probes = rng.standard_normal((P, nx, ny)) + 1j * rng.standard_normal((P, nx, ny))
+        I = np.abs(rng.standard_normal((P, nx, ny))) ** 2 * nx * ny
+
+        def run(n):
+            return oms.ms_gradient_2d(n, probes, 2.0, 1.5, k, dz, I)
+    n = 1 - (rng.uniform(0, 1e-2, shape) + 1j * rng.uniform(0, 1e-3, shape))
+    grad, _ = run(n)
+    h = 1e-6
+    for _ in range(12):
+        idx = tuple(int(rng.integers(0, s)) for s in shape)
+        for part in (1.0, 1j):
+            e = np.zeros(shape, dtype=complex)
+            e[idx] = h * part
+            fd = (run(n + e)[1] - run(n - e)[1]) / (2 * h)
+            an = grad[idx].real if part == 1.0 else grad[idx].imag
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (idx, part, fd, an)
+    assert np.all(grad[..., -1] == 0)

@@ -197,6 +197,21 @@ __global__ void grad3d_kernel(const float2* __restrict__ u, const float2* __rest
   *g = cadd(*g, acc);
 }
 
+// G[z][f] += sum_p conj(a[p][a_z0 + a_zs z][f]) b[p][b_z0 + b_zs z][f]  (planes of N values)
+__global__ void plane_conj_dot_kernel(const float2* __restrict__ a, int64_t a_ps, int a_z0, int a_zs,
+                                      const float2* __restrict__ b, int64_t b_ps, int b_z0, int b_zs, int P,
+                                      int64_t N, float2* __restrict__ G) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int z = blockIdx.y;
+  if (f >= N) return;
+  const float2* pa = a + (int64_t)(a_z0 + a_zs * z) * N + f;
+  const float2* pb = b + (int64_t)(b_z0 + b_zs * z) * N + f;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int p = 0; p < P; ++p) acc = cfma(cconj(pa[(size_t)p * a_ps]), pb[(size_t)p * b_ps], acc);
+  float2* g = G + (size_t)z * N + f;
+  *g = cadd(*g, acc);
+}
+
 // pass 3: rows hold a spectrum in natural order; gather it into the FFT's
 // spectrum order, inverse transform, store natural (not transposed)
 template <int R>
@@ -381,6 +396,21 @@ extern "C" int tp_amplitude_seed2d(const void* d_exit, const void* d_sqrt_i, int
   }
   Ms2dPass a3{w2, (float2*)d_seed, nullptr, nullptr, nullptr, probes, nx, ny, 0, N, N};
   TP_M2_DISPATCH(ny, launch_p3<RV>(a3, st));
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_plane_conj_dot(const void* d_a, int64_t a_pstride, int a_z0, int a_zstep, const void* d_b,
+                                 int64_t b_pstride, int b_z0, int b_zstep, int probes, int nplanes, int64_t N,
+                                 void* d_G, void* stream) {
+  if (!d_a || !d_b || !d_G || probes < 0 || nplanes < 1 || N < 1 || nplanes > 65535) {
+    set_error("tp_plane_conj_dot: invalid arguments (probes=%d, nplanes=%d)", probes, nplanes);
+    return TP_ERR_INVALID;
+  }
+  if (probes == 0) return TP_OK;
+  hipLaunchKernelGGL(plane_conj_dot_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)nplanes), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)d_a, a_pstride, a_z0, a_zstep, (const float2*)d_b,
+                     b_pstride, b_z0, b_zstep, probes, N, (float2*)d_G);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }

@@ -200,3 +200,93 @@ class ExitWaves(torch.autograd.Function):
         Gn[:, 1:] += Gt
         Gn *= (ctx.dz / 4) * (ctx.k / 1j)
         return torch.conj(Gn), None, None, None, None, None, None
+
+
+# ---------------------------------------------------------------------------
+# Multislice: exact gradient of the far-field amplitude misfit.
+# psi_{z+1} = P (T_z psi_z), P = F^-1 H F (ms_solver.py:143-158).  With
+# lambda_L = dJ/d conj(psi_L), mu_z = P^H lambda_{z+1} and lambda_z = conj(T_z) mu_z,
+# so the adjoint is the same multislice sweep with conj(H), started from the
+# seed, whose transmission at step j is conj(T_{L-j}) (1 at step 0): its plane j
+# is mu_{L-j}.  Then dJ/dRe n_z + i dJ/dIm n_z = -2 i k dz conj(T_z) sum_p conj(psi_z) mu_z
+# (tp_plane_conj_dot), and n's last column has zero gradient.
+# ---------------------------------------------------------------------------
+
+
+def _ms_gradient_core(T, u0, run, seed_fn, k: float, dz: float, seed=None):
+    L, N = T.shape
+    P = u0.shape[0]
+    dev = T.device
+    u = torch.empty((P, L + 1, N), dtype=torch.complex64, device=dev)
+    run(T, u0, u, False)
+    ex = u[:, L].contiguous()
+    sd, loss = seed_fn(ex)
+    if seed is not None:
+        sd = _dev_tensor(seed, torch.complex64, dev).reshape(P, N).contiguous()
+    Tadj = torch.empty_like(T)
+    Tadj[0] = 1
+    if L > 1:
+        Tadj[1:] = T.flip(0)[: L - 1].conj()
+    lam = torch.empty_like(u)
+    run(Tadj, sd, lam, True)
+    G = torch.zeros((L, N), dtype=torch.complex64, device=dev)
+    _lib.check(_lib.load().tp_plane_conj_dot(_lib.ptr(u), (L + 1) * N, 0, 1, _lib.ptr(lam), (L + 1) * N, L, -1, P, L,
+                                             N, _lib.ptr(G), _lib.stream_handle()), "tp_plane_conj_dot")
+    grad = torch.zeros((N, L + 1), dtype=torch.complex128, device=dev)
+    grad[:, :L] = ((-2j * k * dz) * (T.conj() * G)).transpose(0, 1).to(torch.complex128)
+    return float(loss.double().sum()), grad
+
+
+def ms_amplitude_gradient(n, probes, sqrt_i, dx: float, k: float, dz: float, n_medium: float = 1.0, seed=None):
+    """(J, grad) for J = sum_p sum (|fft(psi_L)| - sqrt(I))^2 through the multislice,
+    grad = dJ/dRe n + i dJ/dIm n, complex128 (nx, nz) on the GPU.  n (nx, nz), probes
+    (P, nx), sqrt_i (P, nx); ``seed`` (P, nx) replaces the loss seed when given."""
+    from .forward_model import ms_engine
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n_t = _dev_tensor(n, torch.complex128, dev)
+    nx, nz = n_t.shape
+    T = ms_engine.transmission(n_t, k, dz, n_medium)
+    H = ms_engine.propagator(nx, dx, k, dz, dev)
+    u0 = _dev_tensor(probes, torch.complex64, dev).reshape(-1, nx).contiguous()
+    a = _dev_tensor(sqrt_i, torch.float32, dev).reshape(-1, nx).contiguous()
+
+    def run(Tz, x0, out, conj_h):
+        ms_engine.sweep([ms_engine.MsScanSpec(Tz, H, x0, out=out, conj_h=conj_h)])
+
+    J, grad = _ms_gradient_core(T, u0, run, lambda ex: amplitude_seed(ex, a), k, dz, seed)
+    return J, grad
+
+
+def ms_amplitude_gradient_2d(n, probes, sqrt_i, dx: float, dy: float, k: float, dz: float, n_medium: float = 1.0,
+                             seed=None):
+    """Two transverse axes: n (nx, ny, nz), probes / sqrt_i (P, nx, ny), per-probe fft2
+    far field; grad complex128 (nx, ny, nz)."""
+    from .forward_model import ms_engine
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n_t = _dev_tensor(n, torch.complex128, dev)
+    nx, ny, nz = n_t.shape
+    N = nx * ny
+    T = ms_engine.transmission(n_t.reshape(N, nz), k, dz, n_medium)
+    H = ms_engine.propagator_2d(nx, ny, dx, dy, k, dz, dev)
+    u0 = _dev_tensor(probes, torch.complex64, dev).reshape(-1, N).contiguous()
+    P = u0.shape[0]
+    a = _dev_tensor(sqrt_i, torch.float32, dev).reshape(P, N).contiguous()
+    lib = _lib.load()
+    work = torch.empty((3, P, N), dtype=torch.complex64, device=dev)
+
+    def run(Tz, x0, out, conj_h):
+        _lib.check(lib.tp_ms2d_sweep(_lib.ptr(Tz), _lib.ptr(H), _lib.ptr(x0), _lib.ptr(out), None, nx, ny,
+                                     Tz.shape[0], P, int(conj_h), _lib.ptr(work), _lib.stream_handle()),
+                   "tp_ms2d_sweep")
+
+    def seed_fn(ex):
+        sd = torch.empty_like(ex)
+        loss = torch.zeros(P, dtype=torch.float32, device=dev)
+        _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(a), nx, ny, P, 1.0, _lib.ptr(sd), _lib.ptr(loss),
+                                           _lib.ptr(work), _lib.stream_handle()), "tp_amplitude_seed2d")
+        return sd, loss
+
+    J, grad = _ms_gradient_core(T, u0, run, seed_fn, k, dz, seed)
+    return J, grad.reshape(nx, ny, nz)
