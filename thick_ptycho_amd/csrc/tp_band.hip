@@ -68,6 +68,39 @@ __device__ __forceinline__ float2 band_sigma_at(const double2* C, int ncol, int6
   return to_f2(v);
 }
 
+// DPP move of a double (two 32-bit halves) with a row-local control
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over lanes [0, NL) of the wavefront (other lanes must hold 0), uniform
+// result: within each row of 16 lanes quad_perm [1,0,3,2], [2,3,0,1], then
+// row_half_mirror and row_mirror pair every lane with the other quads / half,
+// leaving the row sum in all 16 lanes; rows are then added with readlane.
+template <int NL>
+__device__ __forceinline__ double lanes_sum(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  double t = readlane_d(v, 0);
+  if (NL > 16) t += readlane_d(v, 16);
+  if (NL > 32) t += readlane_d(v, 32);
+  if (NL > 48) t += readlane_d(v, 48);
+  return t;
+}
+
 template <int W>
 __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f) {
   constexpr int BW = 2 * W + 1;
@@ -84,61 +117,90 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
   double edge = 0.0, dmax = 0.0;
   bool singular = false;
 
+  double2* Dg = S + nx * BW;  // [3][nx]: sub, diagonal (+ object term), super of A_jj
   for (int j = 0; j < ny; ++j) {
     const int type = (j == 0) ? 0 : (j == ny - 1 ? 2 : 1);
     const double2* tri = f.tri + (size_t)type * 3 * nx;
     const double2 fe = (j > 0) ? zmul(f.cpl[2 * j], f.cpl[2 * (j - 1) + 1]) : zero;
     const double2 mfe = make_double2(-fe.x, -fe.y);
+    // stage A_jj's three diagonals in LDS: all of a lane's global loads are
+    // issued together instead of one dependent load per band entry
+    for (int i0 = 0; i0 < nx; i0 += 256) {
+      double2 a[4], b[4], c[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = min(i0 + lane + 64 * q, nx - 1);
+        a[q] = tri[i];
+        b[q] = tri[nx + i];
+        c[q] = tri[2 * nx + i];
+        if (lok) {
+          double2 cc = f.C[(size_t)(j * nx + i) * f.ncol + lc];
+          if (f.conj_c) cc = zconj(cc);
+          b[q] = zadd(b[q], zscale(cc, f.lsign));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + lane + 64 * q;
+        if (i < nx) {
+          Dg[i] = a[q];
+          Dg[nx + i] = b[q];
+          Dg[2 * nx + i] = c[q];
+        }
+      }
+    }
+    __syncthreads();
     // S_j = A_jj - f_j e_{j-1} W_{j-1}  (W_{j-1} is still in S)
     for (int t = lane; t < nx * BW; t += 64) {
       const int i = t / BW, d = t - i * BW, col = i + d - W;
       double2 v = zero;
       if (col >= 0 && col < nx) {
         if (j > 0) v = zmul(mfe, S[t]);
-        if (d == W - 1) {
-          v = zadd(v, tri[i]);
-        } else if (d == W) {
-          v = zadd(v, tri[nx + i]);
-          if (lok) {
-            double2 c = f.C[(size_t)(j * nx + i) * f.ncol + lc];
-            if (f.conj_c) c = zconj(c);
-            v = zadd(v, zscale(c, f.lsign));
-          }
-        } else if (d == W + 1) {
-          v = zadd(v, tri[2 * nx + i]);
-        }
+        if (d >= W - 1 && d <= W + 1) v = zadd(v, Dg[(d - W + 1) * nx + i]);
       }
       S[t] = v;
     }
     __syncthreads();
 
-    // banded LDU, no pivoting (the blocks are strictly diagonally dominant)
-    for (int k = 0; k < nx; ++k) {
-      const int m = min(W, nx - 1 - k);
-      const double2 piv = S[k * BW + W];
-      if (piv.x == 0.0 && piv.y == 0.0) singular = true;
-      const double2 inv = zinv(piv);
+    // banded LDU, no pivoting (the blocks are strictly diagonally dominant).
+    // Step k updates the trailing window with the unscaled column / row k; the
+    // scaling of column / row k into L and U is done during step k + 1, which
+    // touches neither, so one barrier per step suffices.
+    double2 inv_prev = zero;
+    int m_prev = 0;
+    for (int k = 0; k <= nx; ++k) {
+      const int m = k < nx ? min(W, nx - 1 - k) : 0;
+      double2 inv = zero;
+      if (k < nx) {
+        const double2 piv = S[k * BW + W];
+        if (piv.x == 0.0 && piv.y == 0.0) singular = true;
+        inv = zinv(piv);
 #pragma unroll
-      for (int u = 0; u < UPD; ++u) {
-        const int t = lane + 64 * u;
-        const int a = t / W, b = t - a * W;
-        if (t < W * W && a < m && b < m) {
-          const int i = k + 1 + a, c = k + 1 + b;
-          const double2 sik = S[i * BW + (k - i + W)];
-          const double2 skc = S[k * BW + (c - k + W)];
-          const int o = i * BW + (c - i + W);
-          S[o] = zsub(S[o], zmul(zmul(sik, skc), inv));
+        for (int u = 0; u < UPD; ++u) {
+          const int t = lane + 64 * u;
+          const int a = t / W, b = t - a * W;
+          if (t < W * W && a < m && b < m) {
+            const int i = k + 1 + a, c = k + 1 + b;
+            const double2 sik = S[i * BW + (k - i + W)];
+            const double2 skc = S[k * BW + (c - k + W)];
+            const int o = i * BW + (c - i + W);
+            S[o] = zsub(S[o], zmul(zmul(sik, skc), inv));
+          }
+        }
+      }
+      if (k > 0) {
+        const int kp = k - 1;
+        if (lane < m_prev) {
+          const int o = (kp + 1 + lane) * BW + (W - 1 - lane);  // L(kp+1+lane, kp)
+          S[o] = zmul(S[o], inv_prev);
+        } else if (lane >= W && lane - W < m_prev) {
+          const int o = kp * BW + (W + 1 + lane - W);  // U(kp, kp+1+lane-W)
+          S[o] = zmul(S[o], inv_prev);
         }
       }
       __syncthreads();
-      if (lane < m) {
-        const int o = (k + 1 + lane) * BW + (W - 1 - lane);  // L(k+1+lane, k)
-        S[o] = zmul(S[o], inv);
-      } else if (lane >= W && lane - W < m) {
-        const int o = k * BW + (W + 1 + lane - W);  // U(k, k+1+lane-W)
-        S[o] = zmul(S[o], inv);
-      }
-      __syncthreads();
+      inv_prev = inv;
+      m_prev = m;
     }
 
     // Takahashi: band of Z = S^{-1}, rows from the bottom.  Lanes [0, W): upper
@@ -162,12 +224,7 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
       const double2 val = make_double2(-acc.x, -acc.y);
       double2 prod = zero;
       if (act && lo) prod = zmul(S[i * BW + (c - i + W)], val);  // U(i, c) Z(c, i)
-      double px = prod.x, py = prod.y;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        px += __shfl_xor(px, off, 64);
-        py += __shfl_xor(py, off, 64);
-      }
+      const double px = lanes_sum<2 * W>(prod.x), py = lanes_sum<2 * W>(prod.y);
       const double2 Di = S[i * BW + W];
       __syncthreads();
       if (act) {
@@ -616,7 +673,7 @@ extern "C" int tp_band_factor(const void* d_C, int nx, int ny, int ncol, const v
               "w in {4, 6, 8, 12, 16, 24, 32}, w < nx)", nx, ny, steps, ncol, w);
     return TP_ERR_INVALID;
   }
-  const size_t lds = (size_t)nx * (2 * w + 1) * sizeof(double2);
+  const size_t lds = (size_t)nx * (2 * w + 4) * sizeof(double2);  // band + three staged diagonals
   if (lds > 163840) {
     set_error("tp_band_factor: band of %d rows x %d diagonals needs %zu bytes of LDS (> 160 KiB)", nx, 2 * w + 1, lds);
     return TP_ERR_UNSUPPORTED;

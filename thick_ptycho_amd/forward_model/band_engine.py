@@ -222,7 +222,7 @@ def factor(C: torch.Tensor, A: Block3D, B: Block3D, mode: str, w: Optional[int] 
     C = C.contiguous()
     plan = band_mode_plan(A, B, mode, L)
     plan.check_complementary()
-    widths = [x for x in BAND_WIDTHS if x >= (w or 6) and x < nx and nx * (2 * x + 1) * 16 <= FACTOR_LDS]
+    widths = [x for x in BAND_WIDTHS if x >= (w or 6) and x < nx and nx * (2 * x + 4) * 16 <= FACTOR_LDS]
     if not widths:
         raise ValueError(f"no band half-width >= {w or 6} below nx = {nx} fits the factor ({BAND_WIDTHS})")
     e = float("inf")
