@@ -393,9 +393,10 @@ def bench_recon(dev, iters: int) -> dict:
     n0 = np.ones((nx, ny, nz), dtype=complex)
     n_true = torch.from_numpy(sc.n).to(dev)
     rc = recon3d.Reconstruction3D(I.sqrt_(), probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
-    res["gradient_ms_by_probes_per_block"] = rc.tune()
-    res["probes_per_block"] = rc.ppb
-    log(f"3D reconstruction: gradient ms by probes per workgroup {res['gradient_ms_by_probes_per_block']}")
+    res["gradient_ms_by_sweep_shape"] = rc.tune()
+    res["probes_per_block"], res["sweep_schedule"] = rc.ppb, rc.schedule
+    log(f"3D reconstruction: gradient ms by sweep shape (probes per workgroup / schedule) "
+        f"{res['gradient_ms_by_sweep_shape']}")
     alpha = 0.5 / rc.lipschitz()
     rc.capture()
     rc.run(1, alpha=alpha, check_band=False)  # warm-up replay

@@ -265,6 +265,11 @@ typedef struct tp_band_scan {
  * probes_per_block (1, 2 or 4) probes, which share each load of the band coefficients,
  * u_s = A(s)^{-1}(sigma u_{s-1} + b_s) - u_{s-1} (sigma = 2 unless given). */
 int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_per_block, void* stream);
+/* tp_band_sweep with a choice of schedule: 0 (the default) prefetches several
+ * blocks' coefficients into registers, one workgroup per CU; 1 prefetches one
+ * block and fits two workgroups on a CU (built for w <= 8 and 1 or 2 probes per
+ * workgroup; TP_ERR_UNSUPPORTED otherwise).  Results are identical. */
+int tp_band_sweep_ex(const tp_band_scan* scans, int nscans, int probes_per_block, int schedule, void* stream);
 
 /* Band factors of A(s)^H from those of A(s) (replaces a second tp_band_factor for
  * the adjoint modes of iterative_lu.py:90-128, whose LHS is the conjugate

@@ -140,6 +140,7 @@ SIGNATURES = {
          c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_int, c_void_p]),
+    "tp_band_sweep_ex": (c_int, [ctypes.POINTER(BandScan), c_int, c_int, c_int, c_void_p]),
     "tp_plane_conj_dot": (c_int, [c_void_p, ctypes.c_int64, c_int, c_int, c_void_p, ctypes.c_int64, c_int, c_int,
                                   c_int, c_int, ctypes.c_int64, c_void_p, c_void_p]),
     "tp_band_conj_transpose": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),

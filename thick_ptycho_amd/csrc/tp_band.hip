@@ -334,12 +334,15 @@ __device__ __forceinline__ float2 row_mv(const BandRow<W>& r, const float2* xb, 
 // stream when many probes share a slice) by PB.  G keeps two banks of G rows
 // within the register budget and one group's loads below the 63 outstanding
 // vector-memory operations the wait counter tracks.
-template <int W, int PB>
+// OCC selects the other schedule: one block per bank and at most 256 VGPRs, so
+// that two workgroups share a CU and hide each other's latency instead.
+template <int W, int PB, bool OCC = false>
 struct BandGroup {
   static constexpr int R = (W + 1) * 4 + PB * 4 + 4;  // VGPRs of one block of one bank
   static constexpr int GP = 100 / R;
-  static constexpr int G = PB == 1 ? (W <= 4 ? 4 : (W <= 6 ? 3 : (W <= 12 ? 2 : 1)))
-                                   : (GP < 1 ? 1 : (GP > 4 ? 4 : GP));
+  static constexpr int G = OCC ? 1
+                               : (PB == 1 ? (W <= 4 ? 4 : (W <= 6 ? 3 : (W <= 12 ? 2 : 1)))
+                                          : (GP < 1 ? 1 : (GP > 4 ? 4 : GP)));
 };
 
 template <int W, int G, int PB>
@@ -483,9 +486,9 @@ __device__ __forceinline__ int64_t band_plane_stride(const tp_band_scan& d) {
   return d.out ? d.out_pstride : d.work_pstride;
 }
 
-template <int W, int PB, bool SIGMA, bool RHS>
-__global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) {
-  constexpr int G = BandGroup<W, PB>::G;
+template <int W, int PB, bool SIGMA, bool RHS, bool OCC>
+__global__ __launch_bounds__(256, OCC ? 2 : 1) void band_sweep_kernel(const BandSweepArgs a) {
+  constexpr int G = BandGroup<W, PB, OCC>::G;
   extern __shared__ float2 xbuf[];  // two exchange buffers of PB x (NT + 2W)
   const int NT = blockDim.x;
   const int blk = band_xcd_block(blockIdx.x, a.nblk);
@@ -584,26 +587,36 @@ __global__ __launch_bounds__(256) void band_sweep_kernel(const BandSweepArgs a) 
   }
 }
 
-template <int W, int PB, bool SIGMA>
+template <int W, int PB, bool SIGMA, bool OCC>
 static int launch_band_sweep(const BandSweepArgs& a, int nt, bool rhs, hipStream_t st) {
   const size_t lds = (size_t)2 * PB * (nt + 2 * W) * sizeof(float2);
   if (rhs)
-    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, true>), dim3(a.nblk), dim3(nt), lds, st, a);
+    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, true, OCC>), dim3(a.nblk), dim3(nt), lds, st, a);
   else
-    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, false>), dim3(a.nblk), dim3(nt), lds, st, a);
+    hipLaunchKernelGGL((band_sweep_kernel<W, PB, SIGMA, false, OCC>), dim3(a.nblk), dim3(nt), lds, st, a);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
 
+// schedule 0: deep prefetch (one workgroup per CU); 1: two workgroups per CU
+// (built for w <= 8 and 1 or 2 probes per workgroup)
 template <int W, bool SIGMA>
-static int launch_band_sweep_pb(const BandSweepArgs& a, int nt, bool rhs, int pb, hipStream_t st) {
+static int launch_band_sweep_pb(const BandSweepArgs& a, int nt, bool rhs, int pb, int schedule, hipStream_t st) {
+  if (schedule == 1) {
+    if constexpr (W <= 8) {
+      if (pb == 1) return launch_band_sweep<W, 1, SIGMA, true>(a, nt, rhs, st);
+      if (pb == 2) return launch_band_sweep<W, 2, SIGMA, true>(a, nt, rhs, st);
+    }
+    set_error("tp_band_sweep_ex: schedule 1 is built for w <= 8 and 1 or 2 probes per workgroup (w=%d, %d)", W, pb);
+    return TP_ERR_UNSUPPORTED;
+  }
   switch (pb) {
     case 1:
-      return launch_band_sweep<W, 1, SIGMA>(a, nt, rhs, st);
+      return launch_band_sweep<W, 1, SIGMA, false>(a, nt, rhs, st);
     case 2:
-      return launch_band_sweep<W, 2, SIGMA>(a, nt, rhs, st);
+      return launch_band_sweep<W, 2, SIGMA, false>(a, nt, rhs, st);
     case 4:
-      return launch_band_sweep<W, 4, SIGMA>(a, nt, rhs, st);
+      return launch_band_sweep<W, 4, SIGMA, false>(a, nt, rhs, st);
   }
   set_error("tp_band_sweep: probes_per_block must be 1, 2 or 4 (got %d)", pb);
   return TP_ERR_INVALID;
@@ -754,7 +767,12 @@ extern "C" int tp_band_sigma(const void* d_C, int nx, int ny, int ncol, double l
   return TP_OK;
 }
 
-extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_per_block, void* stream) {
+extern "C" int tp_band_sweep_ex(const tp_band_scan* scans, int nscans, int probes_per_block, int schedule,
+                                void* stream) {
+  if (schedule != 0 && schedule != 1) {
+    set_error("tp_band_sweep_ex: schedule must be 0 or 1 (got %d)", schedule);
+    return TP_ERR_INVALID;
+  }
   if (!scans || nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("tp_band_sweep: nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
     return TP_ERR_INVALID;
@@ -807,7 +825,8 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_p
   hipStream_t st = (hipStream_t)stream;
 #define TP_BAND_CASE(WV)                                                                           \
   case WV:                                                                                        \
-    return sigma ? launch_band_sweep_pb<WV, true>(a, nt, rhs, pb, st) : launch_band_sweep_pb<WV, false>(a, nt, rhs, pb, st);
+    return sigma ? launch_band_sweep_pb<WV, true>(a, nt, rhs, pb, schedule, st)                     \
+                 : launch_band_sweep_pb<WV, false>(a, nt, rhs, pb, schedule, st);
   switch (w) {
     TP_BAND_CASE(4)
     TP_BAND_CASE(6)
@@ -819,4 +838,8 @@ extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_p
   }
 #undef TP_BAND_CASE
   return TP_ERR_UNSUPPORTED;
+}
+
+extern "C" int tp_band_sweep(const tp_band_scan* scans, int nscans, int probes_per_block, void* stream) {
+  return tp_band_sweep_ex(scans, nscans, probes_per_block, 0, stream);
 }

@@ -362,11 +362,20 @@ class BandScanSpec:
         return s
 
 
-def sweep(specs: Sequence[BandScanSpec], probes_per_block: int = 1, stream=None) -> None:
+def sweep(specs: Sequence[BandScanSpec], probes_per_block: int = 1, stream=None, schedule: int = 0) -> None:
     """All probes of up to eight batches through every slice in one launch;
     ``probes_per_block`` (1, 2, 4) probes share each workgroup and each load of
-    the band coefficients."""
+    the band coefficients; ``schedule`` 1 trades prefetch depth for two
+    workgroups per CU (w <= 8, 1 or 2 probes per workgroup)."""
     assert 1 <= len(specs) <= 8
     arr = (_lib.BandScan * len(specs))(*[sp.struct() for sp in specs])
-    _lib.check(_lib.load().tp_band_sweep(arr, len(specs), int(probes_per_block), _lib.stream_handle(stream)),
-               "tp_band_sweep")
+    _lib.check(_lib.load().tp_band_sweep_ex(arr, len(specs), int(probes_per_block), int(schedule),
+                                            _lib.stream_handle(stream)), "tp_band_sweep_ex")
+
+
+def sweep_shapes(w: int):
+    """(probes_per_block, schedule) pairs built for band half-width w."""
+    shapes = [(1, 0), (2, 0), (4, 0)]
+    if w <= 8:
+        shapes += [(1, 1), (2, 1)]
+    return shapes

@@ -36,6 +36,7 @@ class Reconstruction3D:
         self.nx, self.ny, self.nz, self.N, self.L = nx, ny, nz, nx * ny, nz - 1
         self.k, self.dz = k, dz
         self.ppb = probes_per_block
+        self.schedule = 0
         P = probes.shape[0]
         self.P = P
         self.batch = batch or P
@@ -81,14 +82,15 @@ class Reconstruction3D:
             b1 = min(b0 + self.batch, self.P)
             nb = b1 - b0
             u, lam = self.u[:nb], self.lam[:nb]
-            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb)
+            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb, schedule=self.schedule)
             ex = self.exit[:nb]
             ex.copy_(u[:, self.L])
             _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb, 1.0,
                                                _lib.ptr(self.seed[:nb]), _lib.ptr(self.loss[b0:b1]),
                                                _lib.ptr(self.work), sh()), "tp_amplitude_seed2d")
             rhs = self.seed[:nb] if seed is None else seed[b0:b1].reshape(nb, 1, self.N)
-            be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)], self.ppb)
+            be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)], self.ppb,
+                     schedule=self.schedule)
             _lib.check(lib.tp_grad3d_accumulate(_lib.ptr(u), _lib.ptr(lam), nb, self.L, self.N, _lib.ptr(self.G), sh()),
                        "tp_grad3d_accumulate")
         Gt = self.G.transpose(0, 1).to(torch.complex128)
@@ -115,22 +117,22 @@ class Reconstruction3D:
         self.n.copy_(n0)
         return Lc
 
-    def tune(self, candidates=(1, 2, 4)) -> dict:
-        """Pick the probes per workgroup of the band sweeps by timing one gradient
-        evaluation with each; returns {ppb: ms}."""
+    def tune(self, candidates=None) -> dict:
+        """Pick the band sweeps' shape (probes per workgroup, schedule) by timing one
+        gradient evaluation with each; returns {"ppb/schedule": ms}."""
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         times = {}
-        for ppb in candidates:
-            self.ppb = ppb
+        for shape in candidates or be.sweep_shapes(self.w):
+            self.ppb, self.schedule = shape
             self.gradient()
             e0.record()
             self.gradient()
             e1.record()
             torch.cuda.synchronize()
-            times[ppb] = e0.elapsed_time(e1)
-        self.ppb = min(times, key=times.get)
+            times[shape] = e0.elapsed_time(e1)
+        self.ppb, self.schedule = min(times, key=times.get)
         self.graph = None
-        return times
+        return {f"{p}/{q}": t for (p, q), t in times.items()}
 
     def capture(self) -> None:
         """Capture one iteration (gradient + update) into a HIP graph."""
