@@ -160,7 +160,10 @@ class PWEIterativeLUSolver(BasePWESolver):
 
     def _launch(self, specs, probes_per_block: int = 4, plain: bool = False) -> None:
         if self.dimension == 3:
-            band_engine.sweep(specs)
+            # two probes per workgroup share each band-coefficient load once the
+            # launch has enough probes to keep most CUs busy with pairs
+            total = sum(int(sp.u0.shape[0]) for sp in specs)
+            band_engine.sweep(specs, probes_per_block=2 if total >= 384 else 1)
         elif plain:
             tri_engine.sweep_plain(specs)
         else:
