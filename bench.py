@@ -228,10 +228,11 @@ def main() -> int:
         extras["multislice_gradient_ms"] = t_g * 1e3
         log(f"multislice gradient: {t_g * 1e3:.2f} ms, {extras['multislice_gradient_value']:.4g} steps/s (fwd+adj)")
 
+    cnn = None
     if not args.no_3d:
-        extras["config3"] = bench_3d(dev, args.steps)
+        extras["config3"], cnn = bench_3d(dev, args.steps)
     if not args.no_recon:
-        extras["config4"] = bench_recon(dev, args.recon_iters)
+        extras["config4"] = bench_recon(dev, args.recon_iters, cnn)
 
     cpu, cpu_par = None, None
     if not args.no_cpu:
@@ -365,13 +366,13 @@ def bench_3d(dev, steps: int) -> dict:
         res[f"krylov_{pc}_fallbacks"] = st.fallback_probes
         log(f"3D BiCGSTAB ({pc}): {st.seconds * 1e3:.1f} ms forward sweep, mean iterations "
             f"{res[f'krylov_{pc}_iterations']:.2f}, fallbacks {st.fallback_probes}")
-    return res
+    return res, cnn
 
 
 RECON_FIXTURE = os.path.join(ROOT, "tests", "golden", "recon3d_256x256x128_fp64.npz")
 
 
-def bench_recon(dev, iters: int) -> dict:
+def bench_recon(dev, iters: int, cnn=None) -> dict:
     """3D reconstruction loop on 256 x 256 x 128 with 400 probes: intensities of the
     seeded true object simulated on the GPU, start n0 = 1, ``iters`` steepest-descent
     iterations (forward + exact adjoint banded-LU stepping of every probe, 2D far-field
@@ -399,7 +400,8 @@ def bench_recon(dev, iters: int) -> dict:
            "simulate_ms": e0.elapsed_time(e1)}
     n0 = np.ones((nx, ny, nz), dtype=complex)
     n_true = torch.from_numpy(sc.n).to(dev)
-    rc = recon3d.Reconstruction3D(I.sqrt_(), probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
+    I = I.sqrt_()
+    rc = recon3d.Reconstruction3D(I, probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
     res["gradient_ms_by_sweep_shape"] = rc.tune()
     res["probes_per_block"], res["sweep_schedule"] = rc.ppb, rc.schedule
     log(f"3D reconstruction: gradient ms by sweep shape (probes per workgroup / schedule) "
@@ -445,7 +447,26 @@ def bench_recon(dev, iters: int) -> dict:
             "sample": f"{ref.size} seeded positions of the object",
         }
         log(f"vs FP64 CPU reconstruction (5 iterations): {res['fp64_5iter']}")
+    hist_band = out["loss"]
     del rc
+    if cnn is not None:  # learned-preconditioned Krylov forward step (banded LU: adjoint and fallback)
+        rk = recon3d.Reconstruction3D(I, probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, forward="krylov", cnn=cnn)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok = rk.run(2, alpha=alpha, graph=False)
+        torch.cuda.synchronize()
+        dtk = (time.perf_counter() - t0) / 2
+        its = [np.nanmean(st.iterations) for st in rk.krylov_stats]
+        res["krylov_forward"] = {
+            "iteration_ms": dtk * 1e3,
+            "value": 2 * P * L / dtk,
+            "mean_bicgstab_iterations": float(np.mean(its)),
+            "fallback_probes": int(sum(st.fallback_probes for st in rk.krylov_stats)),
+            "loss_rel_vs_band": float(max(abs(a - b) / b for a, b in zip(ok["loss"], hist_band[:2]))),
+        }
+        log(f"3D reconstruction, CNN-preconditioned BiCGSTAB forward: {dtk * 1e3:.1f} ms per iteration, "
+            f"{res['krylov_forward']}")
+        del rk
     # 2D-FFT multislice of the true object, exit planes of all probes
     reps = 3
     ms_engine.multislice_2d(n_true, probes, sc.dx, sc.dy, sc.k, sc.dz)

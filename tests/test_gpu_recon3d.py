@@ -104,3 +104,27 @@ def test_graph_replay_equals_eager_iteration(gpu):
     hb = b.run(3, alpha=alpha, graph=False)
     assert np.allclose(ha["loss"], hb["loss"], rtol=1e-6)
     assert rel_l2(a.object() - n0, b.object() - n0) < 1e-5
+
+
+@pytest.mark.parametrize("precond", ["none", "cnn"])
+def test_krylov_forward_gradient_matches_band(gpu, precond):
+    """Forward planes from the BiCGSTAB stepper (tolerance 1e-7, banded-LU fallback)
+    give the same gradient as the banded-LU forward sweep."""
+    from thick_ptycho_amd import learned
+
+    nx, ny, nz, P = 64, 64, 8, 4
+    g, n_true, n0, probes, I = _problem(nx, ny, nz, P, seed=7)
+    a = recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet")
+    cnn = learned.CNNPreconditioner(seed=0) if precond == "cnn" else None
+    b = recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet", forward="krylov", cnn=cnn,
+                                 krylov_tol=1e-7)
+    a.gradient()
+    b.gradient()
+    ga, gb = a.grad_object().cpu().numpy(), b.grad_object().cpu().numpy()
+    st = b.krylov_stats[-1]
+    print(f"{precond}: gradient rel-L2 {rel_l2(gb, ga):.2e}, mean iterations {np.nanmean(st.iterations):.2f}, "
+          f"fallback probes {st.fallback_probes}")
+    assert rel_l2(gb, ga) < 1e-3
+    assert abs(float(b.loss.sum()) - float(a.loss.sum())) <= 1e-5 * float(a.loss.sum())
+    with pytest.raises(ValueError):
+        recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet", forward="cg")

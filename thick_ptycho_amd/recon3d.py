@@ -12,6 +12,12 @@ host synchronisation and can be captured once into a HIP graph and replayed.
 The reference reconstructs with NLCG (reconstruction/pwe_reconstructor.py:230-368,
 out of scope here); this loop exercises the same forward / adjoint / gradient
 kernels for two transverse axes.
+
+With ``forward="krylov"`` the forward planes come from the learned-preconditioned
+BiCGSTAB stepper instead (``learned.KrylovStepper`` on the current object, the
+banded LU of a slice only where BiCGSTAB misses the tolerance); the adjoint and
+the gradient stay on the banded LU.  That path synchronises with the host on
+every slice, so its iterations run eagerly (no graph).
 """
 
 from __future__ import annotations
@@ -28,7 +34,8 @@ from .forward_model import band_engine as be
 
 class Reconstruction3D:
     def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
-                 batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None):
+                 batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None,
+                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6):
         dev = device or torch.device("cuda", torch.cuda.current_device())
         self.dev = dev
         n0 = np.asarray(n0)
@@ -62,6 +69,17 @@ class Reconstruction3D:
         self.grad = torch.zeros((self.N, nz), dtype=torch.complex128, device=dev)
         self.alpha = torch.zeros(1, dtype=torch.float64, device=dev)
         self.graph = None
+        if forward not in ("band", "krylov"):
+            raise ValueError("forward must be 'band' or 'krylov'")
+        self.forward = forward
+        self.krylov = None
+        self.krylov_stats = []
+        if forward == "krylov":
+            from . import learned
+
+            self.krylov = learned.KrylovStepper(n0, k, dx, dy, dz, bc, precond="cnn" if cnn is not None else "none",
+                                                cnn=cnn, tol=krylov_tol, device=dev)
+            self.krylov.C = self.C  # the stepper follows the object being reconstructed
 
     def _contribution(self) -> None:
         coef = (self.k / 1j) * (self.n.reshape(self.N, self.nz) - 1)
@@ -76,13 +94,20 @@ class Reconstruction3D:
         self._contribution()
         fwd = self.fwd.run(self.C)
         adj = self.adj.run(self.C)
+        if self.krylov is not None:
+            self.krylov._band = None  # fallback factors of the previous object are stale
         self.G.zero_()
         self.loss.zero_()
         for b0 in range(0, self.P, self.batch):
             b1 = min(b0 + self.batch, self.P)
             nb = b1 - b0
             u, lam = self.u[:nb], self.lam[:nb]
-            be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb, schedule=self.schedule)
+            if self.krylov is not None:
+                planes, st = self.krylov.sweep(self.probes[b0:b1], keep="all")
+                u.copy_(planes)
+                self.krylov_stats.append(st)
+            else:
+                be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb, schedule=self.schedule)
             ex = self.exit[:nb]
             ex.copy_(u[:, self.L])
             _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb, 1.0,
@@ -153,6 +178,7 @@ class Reconstruction3D:
         if alpha is None:
             alpha = 0.5 / self.lipschitz()
         self.alpha.fill_(alpha)
+        graph = graph and self.krylov is None
         if graph and self.graph is None:
             self.capture()
         hist: List[float] = []
