@@ -68,6 +68,8 @@ def test_new_entry_points_reject_bad_arguments():
         == _lib.TP_ERR_INVALID
     assert lib.tp_band_sweep(None, 0, 1, None) == _lib.TP_ERR_INVALID
     assert lib.tp_band_conj_transpose(None, 8, 8, 2, 4, 1, None, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_band_sweep_ex(None, 1, 1, 3, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_plane_conj_dot(None, 0, 0, 1, None, 0, 0, 1, 1, 1, 8, None, None) == _lib.TP_ERR_INVALID
     assert lib.tp_band_sigma(None, 8, 8, 2, -1.0, 1.0, 0, 0, 0, 1, 2, None, None) == _lib.TP_ERR_INVALID
     # Krylov / CNN
     assert lib.tp_bicgstab_solve(1, 1, 1, 8, 8, 2, 1, 1, 3, None, 1e-6, 10, 0, 1, None, None) == _lib.TP_ERR_INVALID
