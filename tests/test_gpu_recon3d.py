@@ -9,6 +9,8 @@
 * the HIP-graph replay of an iteration equals the eager iteration.
 """
 
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -128,3 +130,28 @@ def test_krylov_forward_gradient_matches_band(gpu, precond):
     assert abs(float(b.loss.sum()) - float(a.loss.sum())) <= 1e-5 * float(a.loss.sum())
     with pytest.raises(ValueError):
         recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet", forward="cg")
+
+
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "recon3d_256x256x128_fp64.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="FP64 reconstruction fixture not generated")
+def test_reconstruction_256x256x128_against_fp64_fixture(gpu):
+    """Five steepest-descent iterations of the 256 x 256 x 128, 400-probe workload
+    (intensities simulated on the GPU, n0 = 1, the fixture's step) against the FP64
+    CPU reconstruction (tests/golden/make_recon3d_256x256x128.py): loss history
+    within 1e-3 and the object at 4096 seeded positions within 1e-3 of the update."""
+    fx = np.load(FIXTURE)
+    sc = workloads.config4().scans[0]
+    P = sc.probes.shape[0]
+    probes = torch.from_numpy(sc.probes.reshape(P, -1).astype(np.complex64)).cuda()
+    I = recon3d.simulate_intensity_3d(sc.n, probes, sc.k, sc.dx, sc.dy, sc.dz, sc.bc)
+    n0 = np.ones((sc.nx, sc.ny, sc.nz), dtype=complex)
+    rc = recon3d.Reconstruction3D(I.sqrt_(), probes, n0, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, probes_per_block=2)
+    out = rc.run(5, alpha=float(fx["alpha"]), graph=True)
+    n5 = rc.n.reshape(-1)[torch.from_numpy(fx["idx"]).cuda()].cpu().numpy()
+    upd = np.linalg.norm(n5 - fx["n5"]) / np.linalg.norm(fx["n5"] - 1)
+    loss = np.max(np.abs(np.array(out["loss"]) - fx["loss"]) / fx["loss"])
+    print(f"loss {out['loss']} vs {list(fx['loss'])}; update rel-L2 at the sample {upd:.2e}")
+    assert loss < 1e-3
+    assert upd < 1e-3
