@@ -175,6 +175,7 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
         const double2 piv = S[k * BW + W];
         if (piv.x == 0.0 && piv.y == 0.0) singular = true;
         inv = zinv(piv);
+        if (lane == 0) Dg[k] = inv;  // 1 / D_k for the Takahashi diagonal (Dg is free after the build)
 #pragma unroll
         for (int u = 0; u < UPD; ++u) {
           const int t = lane + 64 * u;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
       double2 prod = zero;
       if (act && lo) prod = zmul(S[i * BW + (c - i + W)], val);  // U(i, c) Z(c, i)
       const double px = lanes_sum<2 * W>(prod.x), py = lanes_sum<2 * W>(prod.y);
-      const double2 Di = S[i * BW + W];
+      const double2 dinv = Dg[i];
       __syncthreads();
       if (act) {
         if (up) {
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(64) void band_factor_kernel(const BandFactorArgs f)
         if (c - i == W) edge = fmax(edge, zabs(val));
       }
       if (lane == 0) {
-        const double2 z = zsub(zinv(Di), make_double2(px, py));
+        const double2 z = zsub(dinv, make_double2(px, py));
         S[i * BW + W] = z;
         dmax = fmax(dmax, zabs(z));
       }
