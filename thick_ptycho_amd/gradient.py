@@ -23,7 +23,7 @@ backward runs the same checkpointed exact adjoint.
 from __future__ import annotations
 
 import math
-from typing import Optional, Tuple
+from typing import Tuple
 
 import numpy as np
 import torch

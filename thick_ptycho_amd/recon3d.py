@@ -22,7 +22,6 @@ every slice, so its iterations run eagerly (no graph).
 
 from __future__ import annotations
 
-import math
 from typing import List, Optional
 
 import numpy as np
