@@ -38,6 +38,28 @@ class TriFactors:
             raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
 
 
+# The per-mode plan and its device copy of the base diagonals depend only on the
+# step operators (A0, B0), the mode and L -- not on the object -- so repeated
+# factorisations (a new object every pass or iteration) reuse them, like the
+# reference's cached step matrices (forward_model.py:47-66).  Keyed by the
+# operator objects themselves (kept alive by the cache), at most 64 entries.
+_PLAN_CACHE: "dict" = {}
+
+
+def _plan_and_base(A: Tridiag, B: Tridiag, mode: str, L: int, dev):
+    key = (id(A), id(B), mode, L, str(dev))
+    hit = _PLAN_CACHE.get(key)
+    if hit is not None and hit[0] is A and hit[1] is B:
+        return hit[2], hit[3]
+    plan: ModePlan = mode_plan(A, B, mode, L)
+    plan.check_complementary()
+    base = torch.from_numpy(plan.base_array().astype(np.complex128)).to(dev)
+    if len(_PLAN_CACHE) >= 64:
+        _PLAN_CACHE.pop(next(iter(_PLAN_CACHE)))
+    _PLAN_CACHE[key] = (A, B, plan, base)
+    return plan, base
+
+
 def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[int] = None,
            stream=None) -> TriFactors:
     """Factor every slice's LHS for ``mode``.
@@ -47,10 +69,8 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
     n, L = C.shape
     C = C.contiguous()
     chunk = chunk or chunk_for(n)
-    plan: ModePlan = mode_plan(A, B, mode, L)
-    plan.check_complementary()
     dev = C.device
-    base = torch.from_numpy(plan.base_array().astype(np.complex128)).to(dev)
+    plan, base = _plan_and_base(A, B, mode, L, dev)
     coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     lib = _lib.load()
