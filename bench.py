@@ -108,11 +108,13 @@ def main() -> int:
     steps_per_pass = 2 * steps_per_dir
 
     def make_specs():
+        # the eight factorisations (scan x direction) in one launch, then the sweeps
+        jobs = [(s["C"], s["A"], s["B"], mode) for mode in ("forward", "adjoint") for s in scans]
+        facs = tri_engine.factor_many(jobs, chunk)
         specs = []
-        for mode, key in (("forward", "out_f"), ("adjoint", "out_a")):
-            for s in scans:
-                fac = tri_engine.factor(s["C"], s["A"], s["B"], mode, chunk=chunk)
-                specs.append(tri_engine.TriScanSpec(fac, s["u0"], out=s[key], flip=(mode == "adjoint")))
+        for (_, _, _, mode), fac, s in zip(jobs, facs, scans + scans):
+            key = "out_f" if mode == "forward" else "out_a"
+            specs.append(tri_engine.TriScanSpec(fac, s["u0"], out=s[key], flip=(mode == "adjoint")))
         return specs
 
     def one_pass(plain: bool = False):

@@ -118,6 +118,21 @@ int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_base,
                   int lhs_col0, int rhs_col0, int col_step, int steps,
                   int chunk, int with_sigma, void* d_coef, int* d_status, void* stream);
 
+/* The arguments of one tp_tri_factor call. */
+typedef struct tp_tri_factor_job {
+  const void* C;
+  const void* base;
+  void* coef;
+  int* status;
+  double lhs_sign, rhs_sign;
+  int n, ncol, steps, conj_c, lhs_col0, rhs_col0, col_step, with_sigma;
+} tp_tri_factor_job;
+
+/* Up to TP_MAX_SCANS independent tp_tri_factor calls (one chunk for all) in one
+ * launch, as the factorisations of every (scan, direction) of a pass
+ * (_construct_solve_cache per scan and mode, iterative_lu.py:66-133). */
+int tp_tri_factor_batch(const tp_tri_factor_job* jobs, int njobs, int chunk, void* stream);
+
 /* One scan of a multi-scan sweep. All strides are in complex elements. */
 typedef struct tp_tri_scan {
   const void* coef;       /* d_coef of this scan (step 1 first)                 */

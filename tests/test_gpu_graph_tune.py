@@ -76,3 +76,26 @@ def test_solver_auto_probes_per_block(gpu, rng):
     got = s.solve(n=n)[0, 0]
     g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=K)
     assert rel_l2(got, pwe.solve(g, "dirichlet", n, probes, "forward")) < 1e-4
+
+
+def test_factor_batch_matches_single(gpu, rng):
+    """tp_tri_factor_batch (one launch for several scans and modes) writes the same
+    factors as one tp_tri_factor call per job."""
+    jobs = []
+    for nx in (200, 96, 320):
+        n = random_object(nx, 16, rng, amp=1e-2)
+        A, B = operators.step_operators_1d(nx, K, 2.0, 4.0, "impedance")
+        coef = (K / 1j) * (n - 1)
+        C = torch.tensor((4.0 / 2) * (coef[:, :-1] + coef[:, 1:]) / 2, device="cuda")
+        for mode in ("forward", "exact_adjoint"):
+            jobs.append((C, A, B, mode))
+    chunk = operators.chunk_for(320)
+    single = [tri_engine.factor(C, A, B, mode, chunk=chunk) for C, A, B, mode in jobs]
+    batch = tri_engine.factor_many(jobs, chunk)
+
+    def written(f):  # slots [L][6][64*chunk]; the sigma slot (5) is written only in sigma modes
+        c = f.coef.view(f.steps, 6, -1)
+        return c if f.sigma else c[:, :5]
+
+    for a, b in zip(single, batch):
+        assert torch.equal(written(a), written(b))

@@ -27,6 +27,25 @@ c_double = ctypes.c_double
 c_size_t = ctypes.c_size_t
 
 
+class TriFactorJob(ctypes.Structure):
+    _fields_ = [
+        ("C", c_void_p),
+        ("base", c_void_p),
+        ("coef", c_void_p),
+        ("status", c_void_p),
+        ("lhs_sign", c_double),
+        ("rhs_sign", c_double),
+        ("n", c_int),
+        ("ncol", c_int),
+        ("steps", c_int),
+        ("conj_c", c_int),
+        ("lhs_col0", c_int),
+        ("rhs_col0", c_int),
+        ("col_step", c_int),
+        ("with_sigma", c_int),
+    ]
+
+
 class TriScan(ctypes.Structure):
     _fields_ = [
         ("coef", c_void_p),
@@ -139,6 +158,7 @@ SIGNATURES = {
         [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int,
          c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
+    "tp_tri_factor_batch": (c_int, [ctypes.POINTER(TriFactorJob), c_int, c_int, c_void_p]),
     "tp_band_sweep": (c_int, [ctypes.POINTER(BandScan), c_int, c_int, c_void_p]),
     "tp_band_sweep_ex": (c_int, [ctypes.POINTER(BandScan), c_int, c_int, c_int, c_void_p]),
     "tp_plane_conj_dot": (c_int, [c_void_p, ctypes.c_int64, c_int, c_int, c_void_p, ctypes.c_int64, c_int, c_int,

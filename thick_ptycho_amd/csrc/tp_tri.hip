@@ -71,12 +71,11 @@ __device__ __forceinline__ Mob mob_normalise(Mob m) {
 constexpr int FACTOR_WAVES = 4;
 
 template <int CH>
-__global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_kernel(const FactorArgs f) {
-  __shared__ Mob sh[FACTOR_WAVES][2][64];
+__device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mob (*sh)[2][64]) {
   constexpr int NP = 64 * CH;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int s = blockIdx.x * FACTOR_WAVES + wave;
+  const int s = blk * FACTOR_WAVES + wave;
   if (s >= f.steps) return;  // whole wavefront; no workgroup barrier below
   const int n = f.n;
   const int row0 = lane * CH;
@@ -178,6 +177,30 @@ __global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_kernel(const Fac
     o[A_GB * NP + m * 64 + lane] = to_f2(gb);
   }
   if (bad && f.status) f.status[0] = 1;
+}
+
+template <int CH>
+__global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_kernel(const FactorArgs f) {
+  __shared__ Mob sh[FACTOR_WAVES][2][64];
+  tri_factor_body<CH>(f, blockIdx.x, sh);
+}
+
+// Several independent factorisations (scans, directions) in one launch: each
+// alone leaves most CUs waiting on one long dependency chain per wavefront.
+struct FactorBatch {
+  FactorArgs jobs[TP_MAX_SCANS];
+  int blk_start[TP_MAX_SCANS + 1];
+  int njobs;
+};
+
+template <int CH>
+__global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_batch_kernel(const FactorBatch b) {
+  __shared__ Mob sh[FACTOR_WAVES][2][64];
+  int j = 0;
+#pragma unroll
+  for (int i = 1; i < TP_MAX_SCANS; ++i)
+    if (i < b.njobs && (int)blockIdx.x >= b.blk_start[i]) j = i;
+  tri_factor_body<CH>(b.jobs[j], blockIdx.x - b.blk_start[j], sh);
 }
 
 // ------------------------------------------------------------------ wave scans
@@ -763,6 +786,60 @@ extern "C" int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_bas
   const dim3 grid((steps + FACTOR_WAVES - 1) / FACTOR_WAVES), block(64 * FACTOR_WAVES);
 #define TP_FACTOR_CASE(CH) \
   case CH: hipLaunchKernelGGL((tri_factor_kernel<CH>), grid, block, 0, st, f); break;
+  switch (chunk) {
+    TP_FACTOR_CASE(1)
+    TP_FACTOR_CASE(2)
+    TP_FACTOR_CASE(4)
+    TP_FACTOR_CASE(8)
+    TP_FACTOR_CASE(16)
+    TP_FACTOR_CASE(32)
+  }
+#undef TP_FACTOR_CASE
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_tri_factor_batch(const tp_tri_factor_job* jobs, int njobs, int chunk, void* stream) {
+  int rc = check_chunk(chunk);
+  if (rc) return rc;
+  if (!jobs || njobs < 1 || njobs > TP_MAX_SCANS) {
+    set_error("tp_tri_factor_batch: njobs must be in [1, %d] (got %d)", TP_MAX_SCANS, njobs);
+    return TP_ERR_INVALID;
+  }
+  FactorBatch b;
+  memset(&b, 0, sizeof(b));
+  int nblk = 0;
+  for (int k = 0; k < njobs; ++k) {
+    const tp_tri_factor_job& q = jobs[k];
+    if (q.n < 1 || q.n > 64 * chunk || q.steps < 1 || q.ncol < 1 || !q.C || !q.base || !q.coef) {
+      set_error("tp_tri_factor_batch: job %d: invalid sizes or null pointer (n=%d, chunk=%d, steps=%d)", k, q.n,
+                chunk, q.steps);
+      return TP_ERR_INVALID;
+    }
+    FactorArgs& f = b.jobs[k];
+    f.C = (const double2*)q.C;
+    f.base = (const double2*)q.base;
+    f.n = q.n;
+    f.ncol = q.ncol;
+    f.steps = q.steps;
+    f.lsign = q.lhs_sign;
+    f.rsign = q.rhs_sign;
+    f.conj_c = q.conj_c;
+    f.lcol0 = q.lhs_col0;
+    f.rcol0 = q.rhs_col0;
+    f.cstep = q.col_step;
+    f.with_sigma = q.with_sigma;
+    f.coef = (float2*)q.coef;
+    f.status = q.status;
+    b.blk_start[k] = nblk;
+    nblk += (q.steps + FACTOR_WAVES - 1) / FACTOR_WAVES;
+  }
+  b.blk_start[njobs] = nblk;
+  b.njobs = njobs;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(nblk), block(64 * FACTOR_WAVES);
+#define TP_FACTOR_CASE(CH) \
+  case CH: hipLaunchKernelGGL((tri_factor_batch_kernel<CH>), grid, block, 0, st, b); break;
   switch (chunk) {
     TP_FACTOR_CASE(1)
     TP_FACTOR_CASE(2)

@@ -84,6 +84,29 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
     return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C])
 
 
+def factor_many(jobs, chunk: int, stream=None) -> list:
+    """``factor`` for up to eight independent (C, A, B, mode) jobs in one launch
+    (tp_tri_factor_batch); returns their TriFactors in order."""
+    assert 1 <= len(jobs) <= _lib.TP_MAX_SCANS
+    facs, arr = [], (_lib.TriFactorJob * len(jobs))()
+    for k, (C, A, B, mode) in enumerate(jobs):
+        assert C.is_cuda and C.dtype == torch.complex128 and C.dim() == 2
+        n, L = C.shape
+        C = C.contiguous()
+        plan, base = _plan_and_base(A, B, mode, L, C.device)
+        coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=C.device)
+        status = torch.zeros(1, dtype=torch.int32, device=C.device)
+        j = arr[k]
+        j.C, j.base, j.coef, j.status = _lib.ptr(C), _lib.ptr(base), _lib.ptr(coef), _lib.ptr(status)
+        j.lhs_sign, j.rhs_sign = plan.lhs_sign, plan.rhs_sign
+        j.n, j.ncol, j.steps, j.conj_c = n, L, L, plan.conj_c
+        j.lhs_col0, j.rhs_col0, j.col_step, j.with_sigma = plan.lhs_col0, plan.rhs_col0, plan.col_step, int(plan.sigma)
+        facs.append(TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C]))
+    _lib.check(_lib.load().tp_tri_factor_batch(arr, len(jobs), chunk, _lib.stream_handle(stream)),
+               "tp_tri_factor_batch")
+    return facs
+
+
 @dataclass
 class TriScanSpec:
     """One batch of probes stepped through ``fac``.
