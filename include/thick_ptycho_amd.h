@@ -304,8 +304,10 @@ int tp_band_sigma(const void* d_C, int nx, int ny, int ncol, double lhs_sign, do
  * ms_solver.py:28-31).  Planes complex64 (nx, ny) row-major, nx and ny powers
  * of two from 64 to 512.
  * ------------------------------------------------------------------------ */
-/* H[ia][ib] = exp(i (kz - k) dz) / (nx ny), kz = sqrt(max(k^2 - kx^2 - ky^2, 0)),
- * kx = 2 pi fftfreq(nx, dx)[ia], ky = 2 pi fftfreq(ny, dy)[ib]; complex64 [nx][ny]. */
+/* H(ia, ib) = exp(i (kz - k) dz) / (nx ny), kz = sqrt(max(k^2 - kx^2 - ky^2, 0)),
+ * kx = 2 pi fftfreq(nx, dx)[ia], ky = 2 pi fftfreq(ny, dy)[ib]; complex64 stored
+ * transposed, [ny][nx] (element (ia, ib) at ib*nx + ia), the order in which
+ * tp_ms2d_sweep's second pass reads it. */
 int tp_ms2d_kernel(int nx, int ny, double dx, double dy, double k, double dz, void* d_H, void* stream);
 /* psi <- ifft2(fft2(psi * T_z) * H) (conj(H) when conj_h) for z = 0..steps-1 on
  * every probe.  d_T: complex64 [steps][nx*ny] (tp_ms_transmission with

@@ -31,7 +31,7 @@ struct Ms2dPass {
   float2* out;        // transposed [L][rows] (passes 1, 2) or natural [rows][L] (pass 3), plane p at out + p * out_pstride
   float* inten;       // far field output [P][L][rows] (pass 2 of the far field)
   const float2* T;    // [rows][L] transmission (pass 1 of a slice) or NULL
-  const float2* H;    // pass 2: H[k_this_axis][row] ([L][rows])
+  const float2* H;    // pass 2: H[row][k_this_axis] ([rows][L])
   int P, rows, L;     // rows per plane and row length (L = 64 R)
   int conj_h;
   int64_t in_pstride, out_pstride;
@@ -106,10 +106,10 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
 #pragma unroll
       for (int m = 0; m < R; ++m) ti[r * L + spec_index<R>(lane, m)] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
     } else {
-      // H is [this-axis frequency][kr]
+      // H is [kr][this-axis frequency]: one row per pass-2 row, coalesced
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        float2 h = a.H[(size_t)spec_index<R>(lane, m) * a.rows + kr];
+        float2 h = a.H[(size_t)kr * L + spec_index<R>(lane, m)];
         if (a.conj_h) h = cconj(h);
         x[m] = cmul(x[m], h);
       }
@@ -250,7 +250,7 @@ __global__ void ms2d_kernel_kernel(int A, int B, double da, double db, double k,
   double s, c;
   sincos((kz - k) * dz, &s, &c);
   const double sc = 1.0 / ((double)A * B);
-  H[(size_t)ia * B + ib] = make_float2((float)(c * sc), (float)(s * sc));
+  H[(size_t)ib * A + ia] = make_float2((float)(c * sc), (float)(s * sc));  // [kb][ka]
 }
 
 // dst[p][i] = src[p][i] for P planes of N values with the given plane strides

@@ -103,8 +103,9 @@ def farfield(exit_w: torch.Tensor, stream=None) -> torch.Tensor:
 
 # ---------------------------------------------------------------- two transverse axes
 def propagator_2d(nx: int, ny: int, dx: float, dy: float, k: float, dz: float, device, stream=None) -> torch.Tensor:
-    """H(kx, ky) / (nx ny), complex64 (nx, ny) in numpy.fft.fftfreq order along both axes."""
-    H = torch.empty((nx, ny), dtype=torch.complex64, device=device)
+    """H(kx, ky) / (nx ny) in numpy.fft.fftfreq order along both axes, stored transposed:
+    complex64 (ny, nx), element [iy, ix] = H(kx[ix], ky[iy]) (the sweep's read order)."""
+    H = torch.empty((ny, nx), dtype=torch.complex64, device=device)
     _lib.check(_lib.load().tp_ms2d_kernel(nx, ny, float(dx), float(dy), float(k), float(dz), _lib.ptr(H),
                                           _lib.stream_handle(stream)), "tp_ms2d_kernel")
     return H
