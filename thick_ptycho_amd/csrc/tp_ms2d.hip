@@ -37,13 +37,17 @@ struct Ms2dPass {
   int64_t in_pstride, out_pstride;
 };
 
+// LDS tile rows are L + 1 values apart, so that the column reads of the
+// transposed store (consecutive r, fixed k) fall into distinct banks.
+__host__ __device__ constexpr int tile_ld(int L) { return L + 1; }
+
 // Transposed store of the workgroup's tile: out[p][k][row0 + r] = tile[r][k]
 __device__ __forceinline__ void store_tile_t(const float2* tile, float2* out, int64_t pstride, int p, int rows, int L,
                                              int row0, int nr, int tid, int nt) {
   float2* o = out + (size_t)p * pstride;
   for (int idx = tid; idx < nr * L; idx += nt) {
     const int k = idx / nr, r = idx - k * nr;
-    o[(size_t)k * rows + row0 + r] = tile[r * L + k];
+    o[(size_t)k * rows + row0 + r] = tile[r * tile_ld(L) + k];
   }
 }
 
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass1_kernel(const Ms2dPas
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
   __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
-  __shared__ float2 tile[RPW * L];
+  __shared__ float2 tile[RPW * tile_ld(L)];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
   const int groups = (a.rows + RPW - 1) / RPW;
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass1_kernel(const Ms2dPas
     }
     fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
-    for (int m = 0; m < R; ++m) tile[r * L + spec_index<R>(lane, m)] = x[m];
+    for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + spec_index<R>(lane, m)] = x[m];
   }
   __syncthreads();
   store_tile_t(tile, a.out, a.out_pstride, p, a.rows, L, row0, nr, threadIdx.x, blockDim.x);
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
   __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
-  __shared__ float2 tile[RPW * L];
+  __shared__ float2 tile[RPW * tile_ld(L)];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
   const int groups = (a.rows + RPW - 1) / RPW;
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
     if (FAR) {
       float* ti = reinterpret_cast<float*>(tile);
 #pragma unroll
-      for (int m = 0; m < R; ++m) ti[r * L + spec_index<R>(lane, m)] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
+      for (int m = 0; m < R; ++m) ti[r * tile_ld(L) + spec_index<R>(lane, m)] = fmaf(x[m].x, x[m].x, x[m].y * x[m].y);
     } else {
       // H is [kr][this-axis frequency]: one row per pass-2 row, coalesced
 #pragma unroll
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
       }
       fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
-      for (int m = 0; m < R; ++m) tile[r * L + lane + 64 * m] = x[m];
+      for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + lane + 64 * m] = x[m];
     }
   }
   __syncthreads();
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
     float* o = a.inten + (size_t)p * a.rows * L;
     for (int idx = threadIdx.x; idx < nr * L; idx += blockDim.x) {
       const int k = idx / nr, r = idx - k * nr;
-      o[(size_t)k * a.rows + row0 + r] = ti[r * L + k];
+      o[(size_t)k * a.rows + row0 + r] = ti[r * tile_ld(L) + k];
     }
   } else {
     store_tile_t(tile, a.out, a.out_pstride, p, a.rows, L, row0, nr, threadIdx.x, blockDim.x);
@@ -141,7 +145,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
   __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
-  __shared__ float2 tile[RPW * L];
+  __shared__ float2 tile[RPW * tile_ld(L)];
   __shared__ float red[M2_WAVES];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass
     }
     fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
-    for (int m = 0; m < R; ++m) tile[r * L + lane + 64 * m] = x[m];
+    for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + lane + 64 * m] = x[m];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
