@@ -149,11 +149,14 @@ class Reconstruction3D:
         for shape in candidates or be.sweep_shapes(self.w):
             self.ppb, self.schedule = shape
             self.gradient()
-            e0.record()
-            self.gradient()
-            e1.record()
-            torch.cuda.synchronize()
-            times[shape] = e0.elapsed_time(e1)
+            best = float("inf")
+            for _ in range(2):  # the better of two evaluations
+                e0.record()
+                self.gradient()
+                e1.record()
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            times[shape] = best
         self.ppb, self.schedule = min(times, key=times.get)
         self.graph = None
         return {f"{p}/{q}": t for (p, q), t in times.items()}
