@@ -83,11 +83,17 @@ __device__ __forceinline__ float2 twid(const float2* tw, int e) {
 // 16 complex values (128 bytes), so that the strided writes of the register
 // stage (lane*R + r) and of the merge stages fall into distinct banks.
 __device__ __forceinline__ constexpr int lp(int i) { return i + (i >> 4); }
+// Wider padding (four slots per 16 values) for the buffer that receives the
+// radix-4, NS = 4 stage of the 256-point transform: its writes
+// (j/4)*16 + j%4 + 4r would meet in 4-way bank conflicts with lp.
+__device__ __forceinline__ constexpr int lp4(int i) { return i + ((i >> 4) << 2); }
+template <int PAD>
+__device__ __forceinline__ constexpr int lpad(int i) { return PAD == 4 ? lp4(i) : lp(i); }
 
 // Stockham stage (Govindaraju et al., SC'08 formulation): sub-transforms of
 // size NS are merged by radix RS.  Butterfly j reads src[j + r*N/RS] and writes
 // dst[(j/NS)*NS*RS + j%NS + r*NS].
-template <int N, int RS, int NS, int SIGN>
+template <int N, int RS, int NS, int SIGN, int PIN = 1, int POUT = 1>
 __device__ __forceinline__ void stage_lds(const float2* src, float2* dst, const float2* tw, int lane) {
   constexpr int B = N / RS;
   constexpr int PER = B / 64;
@@ -97,7 +103,7 @@ __device__ __forceinline__ void stage_lds(const float2* src, float2* dst, const 
     const int j = lane + 64 * t;
     float2 v[RS];
 #pragma unroll
-    for (int r = 0; r < RS; ++r) v[r] = src[lp(j + r * B)];
+    for (int r = 0; r < RS; ++r) v[r] = src[lpad<PIN>(j + r * B)];
     const int jm = j & (NS - 1);
     if (NS > 1) {
 #pragma unroll
@@ -106,7 +112,7 @@ __device__ __forceinline__ void stage_lds(const float2* src, float2* dst, const 
     dft_reg<RS, SIGN>(v);
     const int base = (j / NS) * NS * RS + jm;
 #pragma unroll
-    for (int r = 0; r < RS; ++r) dst[lp(base + r * NS)] = v[r];
+    for (int r = 0; r < RS; ++r) dst[lpad<POUT>(base + r * NS)] = v[r];
   }
 }
 
@@ -142,13 +148,16 @@ __device__ __forceinline__ void stage_from_reg(float2* x, float2* dst, int lane)
 
 // Full FFT of n = 64 R points, register layout in and out; buffers a, b are
 // per-wavefront LDS scratch of ms_bufn<R>() points each (n plus padding).
-template <int R, int SIGN>
+// W4: the wider padding of buffer b for R = 4 (conflict-free, 25 % more LDS);
+// kernels whose occupancy is bound by LDS keep the narrow one.
+template <int R, int SIGN, bool W4 = true>
 __device__ __forceinline__ void fft_wave(float2* x, float2* a, float2* b, const float2* tw, int lane) {
   constexpr int N = 64 * R;
+  constexpr int PB = W4 ? 4 : 1;
   if constexpr (R == 4) {  // 4 * 4 * 4 * 4
     stage_from_reg<4, SIGN>(x, a, lane);
-    stage_lds<N, 4, 4, SIGN>(a, b, tw, lane);
-    stage_lds<N, 4, 16, SIGN>(b, a, tw, lane);
+    stage_lds<N, 4, 4, SIGN, 1, PB>(a, b, tw, lane);
+    stage_lds<N, 4, 16, SIGN, PB, 1>(b, a, tw, lane);
     stage_to_reg<N, 4, SIGN>(a, x, tw, lane);
   } else if constexpr (R == 8) {  // 8 * 8 * 8
     stage_from_reg<8, SIGN>(x, a, lane);
@@ -257,10 +266,10 @@ __device__ __forceinline__ void fft_gen_inv(float2* x, const float2* tw, int lan
 }
 
 // forward (SIGN -1) / inverse (SIGN +1, unnormalised) transform of one wavefront's field
-template <int R, int SIGN>
+template <int R, int SIGN, bool W4 = true>
 __device__ __forceinline__ void fft_any(float2* x, float2* a, float2* b, const float2* tw, int lane) {
   if constexpr (ms_pow2<R>()) {
-    fft_wave<R, SIGN>(x, a, b, tw, lane);
+    fft_wave<R, SIGN, W4>(x, a, b, tw, lane);
   } else if constexpr (SIGN < 0) {
     fft_gen_fwd<R>(x, tw, lane);
   } else {
@@ -269,9 +278,9 @@ __device__ __forceinline__ void fft_any(float2* x, float2* a, float2* b, const f
 }
 
 // LDS ping-pong buffers are needed only by the power-of-two Stockham path
-template <int R>
-constexpr int ms_bufn() {
-  return ms_pow2<R>() ? 64 * R + 4 * R : 1;  // lp(64 R - 1) + 1
+template <int R, bool W4 = true>
+constexpr int ms_bufn() {  // lp4(64 R - 1) + 1 for R = 4 with W4, lp(64 R - 1) + 1 otherwise
+  return ms_pow2<R>() ? (R == 4 && W4 ? 64 * R + 16 * R : 64 * R + 4 * R) : 1;
 }
 
 

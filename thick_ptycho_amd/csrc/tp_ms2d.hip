@@ -13,7 +13,8 @@
 // H = exp(i (kz - k) dz) / (A B), kz = sqrt(max(k^2 - ka^2 - kb^2, 0)) (the
 // one-axis rule of ms_solver.py:58-72 with both transverse frequencies; the
 // reference has no two-axis multislice).  One wavefront transforms one row
-// (tp_fft.h); a workgroup of 4 wavefronts handles RPW consecutive rows.  Plane
+// (tp_fft.h, narrow LDS padding: these kernels' occupancy is LDS-bound); a
+// workgroup of 4 wavefronts handles RPW consecutive rows.  Plane
 // sides: powers of two from 64 to 512.
 #include "tp_fft.h"
 
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass1_kernel(const Ms2dPas
   constexpr int L = 64 * R;
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
-  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R, false>()];
   __shared__ float2 tile[RPW * tile_ld(L)];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass1_kernel(const Ms2dPas
       x[m] = src[lane + 64 * m];
       if (t) x[m] = cmul(x[m], t[lane + 64 * m]);
     }
-    fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+    fft_any<R, -1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
     for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + spec_index<R>(lane, m)] = x[m];
   }
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
   constexpr int L = 64 * R;
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
-  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R, false>()];
   __shared__ float2 tile[RPW * tile_ld(L)];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
     float2 x[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) x[m] = src[lane + 64 * m];
-    fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+    fft_any<R, -1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
     if (FAR) {
       float* ti = reinterpret_cast<float*>(tile);
 #pragma unroll
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass2_kernel(const Ms2dPas
         if (a.conj_h) h = cconj(h);
         x[m] = cmul(x[m], h);
       }
-      fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+      fft_any<R, +1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
       for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + lane + 64 * m] = x[m];
     }
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass
   constexpr int L = 64 * R;
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
-  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R, false>()];
   __shared__ float2 tile[RPW * tile_ld(L)];
   __shared__ float red[M2_WAVES];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass
     float2 x[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) x[m] = src[lane + 64 * m];
-    fft_any<R, -1>(x, buf[wave][0], buf[wave][1], tw, lane);
+    fft_any<R, -1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const float av = amp[(size_t)p * a.rows * L + (size_t)spec_index<R>(lane, m) * a.rows + kr];
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_seed_kernel(const Ms2dPass
       part = fmaf(d, d, part);
       x[m] = cscale(x[m], mag > 0.f ? scale * d / mag : 0.f);
     }
-    fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+    fft_any<R, +1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
     for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + lane + 64 * m] = x[m];
   }
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass3_kernel(const Ms2dPas
   constexpr int L = 64 * R;
   constexpr int RPW = m2_rows<R>();
   __shared__ float2 tw[L];
-  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R>()];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R, false>()];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   build_twiddles(tw, L);
   const int groups = (a.rows + RPW - 1) / RPW;
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass3_kernel(const Ms2dPas
     float2 x[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) x[m] = src[spec_index<R>(lane, m)];
-    fft_any<R, +1>(x, buf[wave][0], buf[wave][1], tw, lane);
+    fft_any<R, +1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
 #pragma unroll
     for (int m = 0; m < R; ++m) dst[lane + 64 * m] = x[m];
   }
