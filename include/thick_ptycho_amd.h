@@ -162,6 +162,12 @@ typedef struct tp_tri_scan {
   void* grad;
   int64_t gfwd_pstride, gfwd_zstride, grad_zstride;
   int gfwd_z0, gfwd_zdir, grad_z0, grad_zdir, grad_skip_row0;
+  /* 0: every probe of the scan steps through `coef`.  Otherwise probe p has its
+   * own factors at coef + p*coef_pstride (complex elements, even), as in the
+   * reference's reduced-domain mode where each scan position sees its own window
+   * of the object; persistent sweep only, with probes_per_block = 1 and no
+   * gradient accumulation. */
+  int64_t coef_pstride;
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe

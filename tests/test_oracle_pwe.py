@@ -228,3 +228,19 @@ def test_reverse_mode_inverts_forward_free_space():
     u = pwe.solve(g, "impedance", n, probe[None], "forward")[0]
     back = pwe.solve(g, "impedance", n, u[:, -1][None], "reverse")[0]
     assert np.max(np.abs(back[:, -1] - probe)) < 1e-8 * np.max(np.abs(probe))
+
+
+def test_reference_scan_windows_example():
+    """Reduced windows of the reference's scan-frame rule, worked by hand for
+    nx = 20, 3 scan points, step 4, pad 8: W = 12, centres 6, 10, 14 (round half
+    to even on 5.5, 9.5, 13.5), windows [0, 11), [4, 15), [8, 19)."""
+    from thick_ptycho_amd import SimulationSpace
+
+    assert SimulationSpace.reference_scan_windows(20, 3, 4, 8) == [(0, 11), (4, 15), (8, 19)]
+    ss = SimulationSpace(20, 6, 1.0, 1.0, num_probes=3, reduced_windows=[(0, 11), (4, 15), (8, 19)])
+    assert ss.solve_reduced_domain and ss.effective_nx == 11 and ss.block_size == 11
+    n = np.arange(20 * 6).reshape(20, 6) + 1.0
+    C = ss.create_object_contribution(n, scan_index=1)
+    assert C.shape == (11, 5)
+    coef = (ss.k / 1j) * (n[4:15] - 1)
+    assert np.allclose(C, (ss.dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2)

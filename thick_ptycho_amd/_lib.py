@@ -80,6 +80,7 @@ class TriScan(ctypes.Structure):
         ("grad_z0", c_int),
         ("grad_zdir", c_int),
         ("grad_skip_row0", c_int),
+        ("coef_pstride", c_int64),
     ]
 
 

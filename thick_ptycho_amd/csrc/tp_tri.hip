@@ -461,7 +461,8 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   const int n = d.n;
   const int L = d.steps;
   const int row0 = lane * CH;
-  const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef);
+  // per-probe factors (one probe per workgroup) or the scan's shared factors
+  const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef) + (PPB == 1 ? (size_t)(act ? p : 0) * (d.coef_pstride / 2) : 0);
   float2* tile = reinterpret_cast<float2*>(smem_v) + NA * NP + wave * TILE;
   // gradient row accumulated over the workgroup's probes (tile_index layout)
   float2* gbuf = reinterpret_cast<float2*>(smem_v) + NA * NP + PPB * TILE;
@@ -688,6 +689,11 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
     }
     if (reinterpret_cast<uintptr_t>(d.coef) & 15) {
       set_error("scan %d: coef must be 16-byte aligned", i);
+      return TP_ERR_INVALID;
+    }
+    if (d.coef_pstride != 0 && (ppb != 1 || d.grad != nullptr || (d.coef_pstride & 1) || d.coef_pstride < 0)) {
+      set_error("scan %d: per-probe factors (coef_pstride = %lld) need probes_per_block = 1, no gradient "
+                "accumulation and an even, positive stride", i, (long long)d.coef_pstride);
       return TP_ERR_INVALID;
     }
     if ((d.grad != nullptr) != (scans[0].grad != nullptr) || ((d.grad != nullptr) && d.gfwd == nullptr)) {

@@ -49,8 +49,8 @@ class BaseForwardModelSolver(ABC):
             self.results_dir = getattr(simulation_space, "results_dir", None)
         self._log = log or _setup_log(verbose)
         self.solve_reduced_domain = simulation_space.solve_reduced_domain
-        if self.solve_reduced_domain:
-            raise NotImplementedError("reduced-domain (per-probe window) solves are not built in thick_ptycho_amd")
+        if self.solve_reduced_domain and simulation_space.dimension != 2:
+            raise NotImplementedError("reduced-domain (per-probe window) solves are built for one transverse axis")
         self.effective_shape = simulation_space.effective_shape
         self.block_size = simulation_space.block_size
         self.nz = simulation_space.nz

@@ -194,6 +194,9 @@ class PWEIterativeLUSolver(BasePWESolver):
         rhs = self._rhs_steps(mode, rhs_block, per_probe_rhs)
         shape = (self.num_projections, A, S, self.nz if keep == "all" else 1, N)
         out = torch.empty(shape, dtype=torch.complex64, device=self.device)
+        if self.solve_reduced_domain:
+            self._solve_reduced(n, mode, pr, rhs, per_probe_rhs, out, keep, plain)
+            return out if keep == "all" else out[..., 0, :]
         specs = []
         for p in range(self.num_projections):
             fac = self._get_or_construct_cache(n=n, mode=mode, proj_idx=p).factors
@@ -210,6 +213,79 @@ class PWEIterativeLUSolver(BasePWESolver):
             torch.cuda.synchronize()
             self._timed(f"stepped {len(specs) * S} probes x {self.nz - 1} slices ({mode})", t0)
         return out if keep == "all" else out[..., 0, :]
+
+    def _solve_reduced(self, n, mode, pr, rhs, per_probe_rhs, out, keep, plain) -> None:
+        """Reduced domain: scan s steps through its own window of the object, so
+        every (projection, scan) has its own factors, rebuilt on every call as in
+        the reference (_get_or_construct_cache, pwe/solvers/base_solver.py:114).
+        The factorisations run eight per launch (tp_tri_factor_batch) into one
+        buffer of per-scan factor sets, and one persistent sweep per (projection,
+        angle) steps every scan through its own set (tp_tri_scan.coef_pstride)."""
+        fm = self.pwe_finite_differences
+        S, L = self.num_probes, self.nz - 1
+        per = tri_engine.coef_elems(L, self.chunk)
+        t0 = time.time()
+        windows = self._reduced_windows()
+        ss = self.simulation_space
+        device_windows = (windows is not None and not getattr(ss, "exact_ref_coeff", False)
+                          and getattr(ss, "empty_space_px", 0) == 0)
+        for p in range(self.num_projections):
+            n_p = self.get_projected_obj(n=n, proj_idx=p)
+            if device_windows:  # C of the whole object once; a window's C is a block of its rows
+                nt = torch.as_tensor(np.ascontiguousarray(n_p)).to(self.device, torch.complex128)
+                coef_n = (ss.k / 1j) * (nt - 1)
+                C_full = ((ss.dz / 2) * (coef_n[:, :-1] + coef_n[:, 1:]) / 2).contiguous()
+
+                def contribution(s):
+                    a, b = windows[s]
+                    return C_full[a:b]
+            else:
+                def contribution(s):
+                    return self.object_contribution_device(n_p, s)
+            coef = torch.empty((S, per), dtype=torch.complex64, device=self.device)
+            fac = None
+            for i in range(0, S, 8):
+                jobs = [(contribution(s), fm.A, fm.B, mode) for s in range(i, min(i + 8, S))]
+                facs = tri_engine.factor_many(jobs, self.chunk, out=[coef[s] for s in range(i, min(i + 8, S))])
+                fac = fac or facs[0]
+            whole = tri_engine.TriFactors(coef.reshape(-1), fac.n, L, self.chunk, mode, fac.b_sign, fac.status,
+                                          fac.sigma, keep=[coef])
+            if plain:  # the plain form shares factors within a scan: one scan per window
+                views = [tri_engine.TriFactors(coef[s], fac.n, L, self.chunk, mode, fac.b_sign, fac.status, fac.sigma)
+                         for s in range(S)]
+            for a in range(self.num_angles):
+                kw = {}
+                if keep == "all":
+                    kw["out"], kw["flip"] = out[p, a], mode == "adjoint"
+                else:
+                    kw["exit"] = out[p, a, :, 0]
+                if plain:
+                    specs = []
+                    for s in range(S):
+                        r = rhs[s:s + 1] if (rhs is not None and per_probe_rhs) else rhs
+                        k2 = {key: (v[s:s + 1] if isinstance(v, torch.Tensor) else v) for key, v in kw.items()}
+                        specs.append(self._spec(views[s], pr[a, s:s + 1], rhs=r, **k2))
+                    for i in range(0, S, 8):
+                        self._launch(specs[i:i + 8], plain=True)
+                else:
+                    spec = self._spec(whole, pr[a], rhs=rhs, coef_pstride=per, **kw)
+                    self._launch([spec], probes_per_block=1)
+        if self.verbose:
+            torch.cuda.synchronize()
+            self._timed(f"stepped {self.num_projections * self.num_angles * S} reduced-domain probes x {L} "
+                        f"slices ({mode})", t0)
+
+    def _reduced_windows(self):
+        """[(xmin, xmax)] per scan: this package's SimulationSpace keeps them in
+        _scan_windows, the reference's in _scan_frame_info[s].reduced_limits_discrete."""
+        ss = self.simulation_space
+        w = getattr(ss, "_scan_windows", None)
+        if w:
+            return list(w)
+        frames = getattr(ss, "_scan_frame_info", None)
+        if frames:
+            return [tuple(int(v) for v in f.reduced_limits_discrete.x) for f in frames]
+        return None
 
     def solve(self, n=None, mode: str = "forward", rhs_block=None, probes=None, **kwargs) -> np.ndarray:
         """Reference-compatible solve (base/base_solver.py:84-154): complex128

@@ -84,9 +84,15 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
     return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C])
 
 
-def factor_many(jobs, chunk: int, stream=None) -> list:
+def coef_elems(L: int, chunk: int) -> int:
+    """Complex elements of one factor set of L slices (tp_tri_coef_bytes / 8)."""
+    return L * 6 * 64 * chunk
+
+
+def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None) -> list:
     """``factor`` for up to eight independent (C, A, B, mode) jobs in one launch
-    (tp_tri_factor_batch); returns their TriFactors in order."""
+    (tp_tri_factor_batch); returns their TriFactors in order.  ``out`` optionally
+    gives each job's coefficient buffer (complex64, coef_elems(L, chunk) values)."""
     assert 1 <= len(jobs) <= _lib.TP_MAX_SCANS
     facs, arr = [], (_lib.TriFactorJob * len(jobs))()
     for k, (C, A, B, mode) in enumerate(jobs):
@@ -94,7 +100,11 @@ def factor_many(jobs, chunk: int, stream=None) -> list:
         n, L = C.shape
         C = C.contiguous()
         plan, base = _plan_and_base(A, B, mode, L, C.device)
-        coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=C.device)
+        if out is not None:
+            coef = out[k]
+            assert coef.dtype == torch.complex64 and coef.is_contiguous() and coef.numel() == coef_elems(L, chunk)
+        else:
+            coef = torch.empty(coef_elems(L, chunk), dtype=torch.complex64, device=C.device)
         status = torch.zeros(1, dtype=torch.int32, device=C.device)
         j = arr[k]
         j.C, j.base, j.coef, j.status = _lib.ptr(C), _lib.ptr(base), _lib.ptr(coef), _lib.ptr(status)
@@ -128,6 +138,7 @@ class TriScanSpec:
     rhs: Optional[torch.Tensor] = None
     flip: bool = False
     rhs_steps: int = 0
+    coef_pstride: int = 0  # per-probe factor sets (reduced domain), see tp_tri_scan
     gfwd: Optional[torch.Tensor] = None
     gfwd_z0: int = 0
     gfwd_zdir: int = 1
@@ -142,6 +153,7 @@ class TriScanSpec:
         n, L = f.n, f.steps
         s = _lib.TriScan()
         s.coef = _lib.ptr(f.coef)
+        s.coef_pstride = int(self.coef_pstride)
         s.sigma = int(f.sigma)
         assert self.u0.dtype == torch.complex64 and self.u0.shape[-1] == n and self.u0.stride(-1) == 1
         s.u0 = _lib.ptr(self.u0)

@@ -13,7 +13,7 @@ objects.
 
 from __future__ import annotations
 
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -35,6 +35,7 @@ class SimulationSpace:
         empty_space_px: int = 0,
         probe_type: str = "gaussian",
         results_dir: Optional[str] = None,
+        reduced_windows: Optional[Sequence[Tuple[int, int]]] = None,
     ):
         self.dimension = 2 if ny is None else 3
         self.nx = nx
@@ -50,14 +51,27 @@ class SimulationSpace:
         self.empty_space_px = empty_space_px
         self.probe_type = probe_type
         self.results_dir = results_dir
-        self.solve_reduced_domain = False
+        # reduced domain: scan s sees the object rows n[xmin_s:xmax_s] (the
+        # reference's _get_reduced_sample, simulation_space_2d.py:132-135)
+        self.solve_reduced_domain = reduced_windows is not None
+        self._scan_windows: List[Tuple[int, int]] = [tuple(map(int, w)) for w in (reduced_windows or [])]
+        if self.solve_reduced_domain:
+            if ny is not None:
+                raise NotImplementedError("reduced-domain solves are built for one transverse axis")
+            if len(self._scan_windows) != num_probes:
+                raise ValueError("reduced_windows needs one (xmin, xmax) window per scan position")
+            widths = {b - a for a, b in self._scan_windows}
+            if len(widths) != 1 or min(a for a, _ in self._scan_windows) < 0 or \
+                    max(b for _, b in self._scan_windows) > nx:
+                raise ValueError("reduced windows must share one width and lie inside [0, nx]")
         self.num_projections = 1
         self.probe_angles = tuple(probe_angles)
         self.num_angles = len(self.probe_angles)
         self.num_probes = num_probes
         self.scan_points = num_probes
         self.total_scans = self.num_angles * self.num_probes * self.num_projections
-        self.effective_nx = nx + 2 * empty_space_px
+        self.effective_nx = (self._scan_windows[0][1] - self._scan_windows[0][0] if self.solve_reduced_domain
+                             else nx) + 2 * empty_space_px
         self.effective_ny = self.ny
         self.z = np.arange(nz) * self.dz
         self.x = np.arange(nx) * self.dx
@@ -73,7 +87,25 @@ class SimulationSpace:
         self.refractive_index_empty = np.ones(self.shape, dtype=complex) * self.n_medium
 
     def _get_reduced_sample(self, n, scan_index):
-        return n
+        if not self.solve_reduced_domain:
+            return n
+        x_min, x_max = self._scan_windows[scan_index]
+        return n[x_min:x_max, :]
+
+    @staticmethod
+    def reference_scan_windows(nx: int, scan_points: int, step_size: int, pad_discrete: int) -> List[Tuple[int, int]]:
+        """The reduced windows of SimulationSpace2D._generate_scan_frames
+        (simulation_space_2d.py:80-130) for scan_points > 1: window width
+        W = step + pad, centres round(mid + (i - (S-1)/2) step), xmin = max(c - W//2, 0),
+        xmax = min(xmin + W - 1, nx - 1); the sample is n[xmin:xmax] (W - 1 rows)."""
+        W = step_size + pad_discrete
+        half, mid = W // 2, (nx - 1) / 2.0
+        out = []
+        for i in range(scan_points):
+            cx = int(round(mid + (i - (scan_points - 1) / 2) * step_size))
+            xmin = max(cx - half, 0)
+            out.append((xmin, min(xmin + W - 1, nx - 1)))
+        return out
 
     def create_object_contribution(self, n=None, grad=False, scan_index=0) -> np.ndarray:
         """Half-step object coefficients C[..., j-1] = (dz/2)(coef[..., j-1] + coef[..., j])/2.
@@ -84,6 +116,8 @@ class SimulationSpace:
         if n is None:
             n = self.refractive_index_empty
         n = np.asarray(n)
+        if self.solve_reduced_domain:
+            n = self._get_reduced_sample(n, scan_index)
         if grad:
             coef = (self.k / 1j) * np.ones_like(n) if not self.exact_ref_coeff else (self.k / 1j) * n
         else:
