@@ -108,3 +108,35 @@ def test_exact_gradient_3d_central_differences():
         fd = (loss(npl) - loss(nmi)) / (2 * h)
         an = grad[idx].real if i % 2 == 0 else grad[idx].imag
         assert abs(fd - an) <= 1e-5 * scale + 1e-4 * abs(fd), (idx, fd, an)
+
+
+def test_simulated_data_file_round_trip(tmp_path):
+    """The reference's simulated_data.npz keys (generate_data.py:193-197)."""
+    from thick_ptycho_amd import reconstruct
+
+    rng = np.random.default_rng(0)
+    I = rng.random((4, 16))
+    ex = rng.standard_normal((4, 16)) + 1j * rng.standard_normal((4, 16))
+    path = str(tmp_path / "simulated_data.npz")
+    reconstruct.save_simulated_data(path, I, ex)
+    with np.load(path, allow_pickle=False) as f:
+        assert sorted(f.files) == ["exit_waves", "intensity"]
+    back = reconstruct.load_simulated_data(path)
+    assert np.array_equal(back["intensity"], I) and np.array_equal(back["exit_waves"], ex)
+
+
+@pytest.mark.parametrize("g_new, g_old, beta", [
+    ((0.5, 0.0), (1.0, 0.0), 0.0),   # PR = -0.25 clipped at zero
+    ((1.0, 1.0), (1.0, 0.0), 1.0),   # PR = 1 below FR = 2
+    ((2.0, 0.0), (1.0, 0.0), 2.0),   # PR = 2 below FR = 4
+    ((-1.0, 0.0), (1.0, 0.0), 1.0),  # PR = 2 capped by FR = 1
+    ((1.0, 0.0), (0.0, 0.0), 0.0),   # zero previous gradient
+])
+def test_nlcg_beta_rule(g_new, g_old, beta):
+    """min(FR, max(PR, 0)) as compute_betak (pwe_reconstructor.py:221-228)."""
+    import torch
+
+    from thick_ptycho_amd import reconstruct
+
+    got = reconstruct.beta_rule(torch.tensor(g_new, dtype=torch.float64), torch.tensor(g_old, dtype=torch.float64))
+    assert got == pytest.approx(beta)

@@ -1,0 +1,124 @@
+"""One-axis reconstruction driver and the reference's data files.
+
+Data files (numpy ``.npz``, the keys the reference writes):
+
+* simulated data, ``np.savez_compressed(..., intensity=..., exit_waves=...)``
+  (generate_data.py:193-197);
+* reconstructions, ``np.savez_compressed(..., n_est=..., wave_field=...,
+  residual_history=...)`` (reconstruct_sample.py:105-108).
+
+The driver minimises J(n) = sum_p sum (|fft(u_L^p)| - sqrt(I^p))^2 with the
+exact gradient of ``gradient.amplitude_gradient`` (checkpointed forward and
+exact-adjoint sweeps on the GPU).  ``method="nlcg"`` is the nonlinear conjugate
+gradient iteration of the reference's ReconstructorPWE.reconstruct
+(pwe_reconstructor.py:230-368): separate directions for Re n and Im n, each with
+beta = min(beta_FR, max(beta_PR, 0)) (compute_betak, pwe_reconstructor.py:221-228).
+The step differs: 0.5 / L with L a secant estimate of the gradient's Lipschitz
+constant instead of the reference's linearised line search (compute_alphak,
+which needs a forward solve per probe and direction), halved and restarted
+along -grad whenever an iterate raises J.  ``method="sd"`` is steepest descent
+with the same step.
+"""
+
+from __future__ import annotations
+
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import gradient as tg
+
+
+def save_simulated_data(path: str, intensity, exit_waves) -> None:
+    np.savez_compressed(path, intensity=np.asarray(intensity), exit_waves=np.asarray(exit_waves))
+
+
+def load_simulated_data(path: str) -> Dict[str, np.ndarray]:
+    with np.load(path, allow_pickle=False) as f:
+        return {"intensity": f["intensity"], "exit_waves": f["exit_waves"]}
+
+
+def save_reconstruction(path: str, n_est, wave_field, residual_history) -> None:
+    np.savez_compressed(path, n_est=np.asarray(n_est), wave_field=np.asarray(wave_field),
+                        residual_history=np.asarray(residual_history))
+
+
+def load_reconstruction(path: str) -> Dict[str, np.ndarray]:
+    with np.load(path, allow_pickle=False) as f:
+        return {k: f[k] for k in ("n_est", "wave_field", "residual_history")}
+
+
+def beta_rule(g_new: torch.Tensor, g_old: torch.Tensor) -> float:
+    """min(FR, max(PR, 0)) for one real gradient: FR = |g_new|^2 / |g_old|^2,
+    PR = g_new . (g_new - g_old) / |g_old|^2 (pwe_reconstructor.py:221-228)."""
+    den = float(torch.dot(g_old.flatten(), g_old.flatten()))
+    if den <= 0:
+        return 0.0
+    fr = float(torch.dot(g_new.flatten(), g_new.flatten())) / den
+    pr = float(torch.dot(g_new.flatten(), (g_new - g_old).flatten())) / den
+    return min(fr, max(pr, 0.0))
+
+
+class Reconstruction2D:
+    """Reconstruct n (nx, nz) from far-field intensities I (P, nx) of probes (P, nx)."""
+
+    def __init__(self, intensity, probes, k: float, dx: float, dz: float, bc: str = "dirichlet",
+                 checkpoint: int = 32, device=None):
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.sqrt_i = torch.as_tensor(np.sqrt(np.asarray(intensity, dtype=float))).to(self.dev, torch.float32)
+        self.probes = torch.as_tensor(np.asarray(probes)).to(self.dev, torch.complex64)
+        self.k, self.dx, self.dz, self.bc, self.checkpoint = k, dx, dz, bc, checkpoint
+
+    def gradient(self, n: torch.Tensor):
+        return tg.amplitude_gradient(n, self.probes, self.sqrt_i, self.k, self.dx, self.dz, self.bc,
+                                     self.checkpoint, device=self.dev)
+
+    def wave_field(self, n) -> np.ndarray:
+        """Forward field of every probe at slices 1..nz-1, complex64 (1, P, nx, nz-1): the
+        reference's ``convert_to_tensor_form(uk)`` layout (base_reconstructor.py:123-140)."""
+        n_t = tg._dev_tensor(n, torch.complex128, self.dev)
+        N, nz = n_t.shape
+        A, B = tg._operators(N, self.k, self.dx, self.dz, self.bc)
+        sw = tg._Sweeper(tg.object_contribution(n_t, self.k, self.dz), A, B, "exact_adjoint", 1)
+        ck, _ = sw.forward(self.probes.reshape(-1, N))
+        return ck[:, 1:].transpose(1, 2).unsqueeze(0).cpu().numpy()
+
+    def lipschitz(self, n: torch.Tensor, h: float = 1e-5) -> float:
+        _, g0 = self.gradient(n)
+        d = (h / g0.abs().max()) * g0
+        _, g1 = self.gradient(n - d)
+        return float(torch.linalg.vector_norm(g1 - g0) / torch.linalg.vector_norm(d))
+
+    def run(self, n0, iters: int, method: str = "nlcg", alpha: Optional[float] = None,
+            wave_field: bool = True) -> dict:
+        """Returns {'n_est': complex128 (nx, nz), 'wave_field': ``wave_field(n_est)``,
+        'residual_history': [J before the first iteration and after each accepted
+        one], 'alpha': the last step}; the three arrays are what
+        ``save_reconstruction`` writes."""
+        if method not in ("nlcg", "sd"):
+            raise ValueError("method must be 'nlcg' or 'sd'")
+        n = torch.as_tensor(np.asarray(n0)).to(self.dev, torch.complex128).clone()
+        if alpha is None:
+            alpha = 0.5 / self.lipschitz(n)
+        J, g = self.gradient(n)
+        d = -g
+        hist: List[float] = [J]
+        for _ in range(iters):
+            n_new = n + alpha * d
+            J_new, g_new = self.gradient(n_new)
+            if J_new > J and method == "nlcg":  # safeguard: smaller step along -grad
+                alpha *= 0.5
+                d = -g
+                continue
+            if method == "nlcg":
+                b_re = beta_rule(g_new.real, g.real)
+                b_im = beta_rule(g_new.imag, g.imag)
+                d = -g_new + torch.complex(b_re * d.real, b_im * d.imag)
+            else:
+                d = -g_new
+            n, J, g = n_new, J_new, g_new
+            hist.append(J)
+        n_est = n.cpu().numpy()
+        return {"n_est": n_est, "wave_field": self.wave_field(n) if wave_field else None,
+                "residual_history": hist, "alpha": alpha}
