@@ -257,6 +257,30 @@ def main() -> int:
         log(f"reduced domain: {t_r * 1e3:.2f} ms per pass, {extras['reduced_domain']['value']:.4g} steps/s")
         del sol_r
 
+        # parallel in time: whole-system GMRES with the alpha-circulant preconditioner
+        from thick_ptycho_amd.forward_model.pint import PWEFullPinTSolver
+
+        extras["pint"] = []
+        for nx_p, nz_p, P_p in ((256, 256, 16), (1024, 512, 8)):
+            ss_p = SimulationSpace(nx_p, nz_p, 2.0, 4.0, num_probes=P_p)
+            n_p = workloads.blob_object_2d(nx_p, nz_p, 12, np.random.default_rng(60))
+            pr_p = workloads.gaussian_probes_1d(nx_p, np.linspace(nx_p / 4, 3 * nx_p / 4, P_p), nx_p / 16)
+            sol_p = PWEFullPinTSolver(ss_p, pr_p[None], bc_type="impedance", warm_start=False)
+            sol_p.solve_device(n=n_p)  # operator and preconditioner factors (cached per mode)
+            t_p = timed(lambda: sol_p.solve_device(n=n_p), 2, 1)
+            res = sol_p.last_result
+            extras["pint"].append({
+                "workload": f"PiT GMRES, {nx_p}x{nz_p} x {P_p} probes, alpha 1e-2, rtol 1e-5, impedance, forward",
+                "value": P_p * (nz_p - 1) / t_p,
+                "ms_per_solve": t_p * 1e3,
+                "iterations": res.iterations,
+                "cycles": res.cycles,
+                "converged": bool(res.converged.all()),
+            })
+            log(f"PiT {nx_p}x{nz_p}x{P_p}: {t_p * 1e3:.2f} ms per solve, {res.iterations} iterations, "
+                f"{extras['pint'][-1]['value']:.4g} steps/s")
+            del sol_p
+
     cnn = None
     if not args.no_3d:
         extras["config3"], cnn = bench_3d(dev, args.steps)

@@ -57,6 +57,13 @@
  *                      (learned-preconditioned) alternative to the per-slice
  *                      LU solve of iterative_lu.py:185-193 in 3D, with the
  *                      banded factor above as its fallback.
+ *   tp_pint_dots, tp_pint_combine
+ *                   <- the Arnoldi projections inside scipy's gmres as called
+ *                      at full_pint.py:240-248.
+ *   tp_pint_matvec  <- _pintobj_matvec_exact (forward_model/pwe/solvers/utils/
+ *                      _pint_utils.py:27-54), the operator of PWEFullPinTSolver
+ *                      (full_pint.py:71-275); its alpha-circulant preconditioner
+ *                      runs on tp_tri_factor / tp_tri_sweep.
  */
 #ifndef THICK_PTYCHO_AMD_H
 #define THICK_PTYCHO_AMD_H
@@ -399,6 +406,33 @@ int tp_bicgstab_solve(const void* d_diag, const void* d_tri32, const void* d_cpl
                       int check_every, void* d_work, int* d_done, void* stream);
 /* out = xscale * x - uprev over `total` complex64 values (the CN update). */
 int tp_cn_finish(int64_t total, const void* d_x, const void* d_uprev, void* d_out, double xscale, void* stream);
+
+/* ---------------------------------------------------------------- parallel in time
+ * Whole space-z system of one transverse axis applied matrix-free, for every
+ * probe: (M x)_j = (A - diag C_j) x_j - (B + diag C_j) x_{j-1}, x_{-1} = 0, and
+ * y = M x, or y = b - M x when d_b is non-null (b may alias y; x may not).
+ * Replaces _pintobj_matvec_exact (forward_model/pwe/solvers/utils/_pint_utils.py:27-54)
+ * in its forward form; the reference's adjoint-mode system passes A^H, B^H and the
+ * conjugated, slice-reversed C in the same form (full_pint.py:148-160).
+ *   d_ops: A sub, diag, sup, B sub, diag, sup (6 x N complex; sub[i] multiplies
+ *          x[i-1], sup[i] multiplies x[i+1]);  d_C: (L, N) complex, row j = C_j;
+ *   d_x, d_b, d_y: (P, L, N) complex.
+ * Complex128 throughout when fp64 = 1, complex64 when 0. */
+int tp_pint_matvec(const void* d_ops, const void* d_C, const void* d_x, const void* d_b, void* d_y, int N, int L,
+                   int P, int fp64, void* stream);
+/* Arnoldi projections of a batch of probes, the Krylov-basis work of the GMRES
+ * call at full_pint.py:240-248 (scipy.sparse.linalg.gmres).  V: (P, rows, len)
+ * complex64 with probe stride v_pstride and row stride v_rstride; w: (P, len).
+ * tp_pint_dots writes partial[(p * nblk + b) * 33 + i] = block b's share of
+ *   sum_e conj(V[p, i, e]) w[p, e] for i < nrows (nrows <= 32) and of
+ *   sum_e |w[p, e]|^2 (real part) in slot nrows; the caller sums over b.
+ * tp_pint_combine writes out[p, e] = w[p, e] - sum_{i < nrows} V[p, i, e] c[p, i]
+ *   (w null: 0; w may alias out). */
+int tp_pint_dots(const void* d_V, int64_t v_pstride, int64_t v_rstride, int nrows, const void* d_w,
+                 int64_t w_pstride, int64_t len, int P, int nblk, void* d_partial, void* stream);
+int tp_pint_combine(const void* d_V, int64_t v_pstride, int64_t v_rstride, int nrows, const void* d_c, int c_pstride,
+                    const void* d_w, int64_t w_pstride, void* d_out, int64_t out_pstride, int64_t len, int P,
+                    void* stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,12 @@ def test_new_entry_points_reject_bad_arguments():
     assert lib.tp_band_sweep_ex(None, 1, 1, 3, None) == _lib.TP_ERR_INVALID
     assert lib.tp_plane_conj_dot(None, 0, 0, 1, None, 0, 0, 1, 1, 1, 8, None, None) == _lib.TP_ERR_INVALID
     assert lib.tp_band_sigma(None, 8, 8, 2, -1.0, 1.0, 0, 0, 0, 1, 2, None, None) == _lib.TP_ERR_INVALID
+    # parallel in time: null operands, x aliasing y
+    assert lib.tp_pint_matvec(None, 1, 1, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_pint_matvec(1, 1, 2, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
+    assert b"alias" in lib.tp_last_error()
+    assert lib.tp_pint_dots(1, 0, 0, 33, 1, 0, 10, 1, 1, 1, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_pint_combine(None, 0, 0, 2, None, 0, None, 0, 1, 0, 10, 1, None) == _lib.TP_ERR_INVALID
     # Krylov / CNN
     assert lib.tp_bicgstab_solve(1, 1, 1, 8, 8, 2, 1, 1, 3, None, 1e-6, 10, 0, 1, None, None) == _lib.TP_ERR_INVALID
     assert lib.tp_cnn_apply(1, 1, 1, 0, 8, 8, 1, None) == _lib.TP_ERR_INVALID
