@@ -415,11 +415,13 @@ int tp_cn_finish(int64_t total, const void* d_x, const void* d_uprev, void* d_ou
  * in its forward form; the reference's adjoint-mode system passes A^H, B^H and the
  * conjugated, slice-reversed C in the same form (full_pint.py:148-160).
  *   d_ops: A sub, diag, sup, B sub, diag, sup (6 x N complex; sub[i] multiplies
- *          x[i-1], sup[i] multiplies x[i+1]);  d_C: (L, N) complex, row j = C_j;
+ *          x[i-1], sup[i] multiplies x[i+1]);  d_C: (L, N) complex, row j = C_j,
+ *          shared by every probe (c_pstride = 0) or one block per probe at
+ *          p * c_pstride (c_pstride >= L * N: reduced-domain windows);
  *   d_x, d_b, d_y: (P, L, N) complex.
  * Complex128 throughout when fp64 = 1, complex64 when 0. */
-int tp_pint_matvec(const void* d_ops, const void* d_C, const void* d_x, const void* d_b, void* d_y, int N, int L,
-                   int P, int fp64, void* stream);
+int tp_pint_matvec(const void* d_ops, const void* d_C, int64_t c_pstride, const void* d_x, const void* d_b, void* d_y,
+                   int N, int L, int P, int fp64, void* stream);
 /* Arnoldi projections of a batch of probes, the Krylov-basis work of the GMRES
  * call at full_pint.py:240-248 (scipy.sparse.linalg.gmres).  V: (P, rows, len)
  * complex64 with probe stride v_pstride and row stride v_rstride; w: (P, len).

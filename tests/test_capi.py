@@ -72,9 +72,10 @@ def test_new_entry_points_reject_bad_arguments():
     assert lib.tp_plane_conj_dot(None, 0, 0, 1, None, 0, 0, 1, 1, 1, 8, None, None) == _lib.TP_ERR_INVALID
     assert lib.tp_band_sigma(None, 8, 8, 2, -1.0, 1.0, 0, 0, 0, 1, 2, None, None) == _lib.TP_ERR_INVALID
     # parallel in time: null operands, x aliasing y
-    assert lib.tp_pint_matvec(None, 1, 1, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
-    assert lib.tp_pint_matvec(1, 1, 2, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_pint_matvec(None, 1, 0, 1, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_pint_matvec(1, 1, 0, 2, None, 2, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID
     assert b"alias" in lib.tp_last_error()
+    assert lib.tp_pint_matvec(1, 1, 31, 2, None, 3, 8, 4, 1, 0, None) == _lib.TP_ERR_INVALID  # c_pstride < L * N
     assert lib.tp_pint_dots(1, 0, 0, 33, 1, 0, 10, 1, 1, 1, None) == _lib.TP_ERR_INVALID
     assert lib.tp_pint_combine(None, 0, 0, 2, None, 0, None, 0, 1, 0, 10, 1, None) == _lib.TP_ERR_INVALID
     # Krylov / CNN

@@ -158,3 +158,41 @@ def test_arnoldi_kernels_match_numpy(gpu, nrows, LN, P, rng):
         assert rel_l2(zero, -np.einsum("pie,pi->pe", V[:, :nrows], c[:, :nrows])) < 1e-5
     else:
         assert np.all(zero == 0)
+
+
+@pytest.mark.parametrize("mode", ["forward", "adjoint"])
+def test_reduced_domain_windows(gpu, mode):
+    """Every scan on its own window of the object (one object term per probe in
+    the batch) against the whole-system solve of each window."""
+    rng = np.random.default_rng(8)
+    nx, nz, S, step = 300, 40, 11, 24
+    windows = SimulationSpace.reference_scan_windows(nx, S, step, nx - S * step)
+    ss = SimulationSpace(nx, nz, DX, DZ, wavelength=2 * np.pi / K, num_probes=S, reduced_windows=windows)
+    n = random_object(nx, nz, rng, amp=1e-2)
+    W = ss.effective_nx
+    probes = workloads.gaussian_probes_1d(W, [W / 2] * S, 10.0)
+    s = gpi.PWEFullPinTSolver(ss, probes[None], bc_type="impedance", rtol=1e-10, atol=0.0)
+    got = s.solve(n=n, mode=mode)[0, 0]
+    assert got.shape == (S, W, nz) and s.last_result.converged.all()
+    g = _grid(W, nz)
+    for i, (a, b) in enumerate(windows):
+        if mode == "forward":
+            ref = pwe.full_lu_solve(g, "impedance", n[a:b], probes[i])
+        else:
+            ref, info, _ = opi.pit_solve(g, "impedance", n[a:b], probes[i], mode="adjoint", rtol=1e-13, atol=0)
+            assert info == 0
+        assert rel_l2(got[i], ref) < 1e-7, (i, a, b)
+
+
+def test_petsc_form_reaches_absolute_tolerance(gpu):
+    nx, nz, P = 256, 64, 2
+    rng = np.random.default_rng(9)
+    n = workloads.blob_object_2d(nx, nz, 8, rng, sigma=(3.0, 10.0), delta=(0, 5e-3), beta=(0, 5e-5))
+    probes = workloads.gaussian_probes_1d(nx, [100.0, 150.0], 20.0)
+    ss = SimulationSpace(nx, nz, DX, DZ, wavelength=2 * np.pi / K, num_probes=P)
+    s = gpi.PWEPetscFullPinTSolver(ss, probes[None], bc_type="impedance", atol=1e-6)
+    got = s.solve(n=n)[0, 0]
+    assert s.last_result.converged.all() and s.last_result.residual.max() <= 1e-6
+    g = _grid(nx, nz)
+    for p in range(P):
+        assert rel_l2(got[p], pwe.full_lu_solve(g, "impedance", n, probes[p])) < 1e-5

@@ -9,7 +9,7 @@
 // contiguous row and a thread per (probe, slice, row) reads its three-point
 // stencils from the same and the previous row of the wavefront's neighbours.
 // HBM-bound: per element it reads x once (the neighbours and the previous slice
-// come from L1/L2), C once per probe and writes y.  FP64 for the residual of the
+// come from L1/L2) and C (from L2 when the probes share it) and writes y.  FP64 for the residual of the
 // refinement loop, FP32 inside the Krylov cycles.
 #include "tp_common.h"
 
@@ -27,17 +27,19 @@ __device__ __forceinline__ double2 add2(double2 a, double2 b) { return zadd(a, b
 
 // ops: A sub, A diag, A sup, B sub, B diag, B sup, each N values (sub[0] and
 // sup[N-1] are not read).
+// C: (L, N) shared by all probes (c_pstride 0) or one (L, N) block per probe
+// (c_pstride = L * N: reduced-domain windows).
 template <typename T>
 __global__ __launch_bounds__(256) void pint_matvec_kernel(const T* __restrict__ ops, const T* __restrict__ C,
-                                                          const T* __restrict__ x, const T* b, T* y, int N, int L,
-                                                          int64_t total) {
+                                                          int64_t c_pstride, const T* __restrict__ x, const T* b, T* y,
+                                                          int N, int L, int64_t total) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= total) return;
   const int i = (int)(t % N);
   const int64_t row = t / N;  // p * L + j
   const int j = (int)(row % L);
   const T* xj = x + row * N;
-  const T c = C[(int64_t)j * N + i];
+  const T c = C[(row / L) * c_pstride + (int64_t)j * N + i];
   const T xc = xj[i];
   T acc = mac(ops[N + i], xc, zero_of(xc));
   if (i > 0) acc = mac(ops[i], xj[i - 1], acc);
@@ -169,9 +171,10 @@ extern "C" int tp_pint_combine(const void* d_V, int64_t v_pstride, int64_t v_rst
   return TP_OK;
 }
 
-extern "C" int tp_pint_matvec(const void* d_ops, const void* d_C, const void* d_x, const void* d_b, void* d_y, int N,
-                              int L, int P, int fp64, void* stream) {
-  if (!d_ops || !d_C || !d_x || !d_y || N < 2 || L < 1 || P < 0 || (fp64 != 0 && fp64 != 1)) {
+extern "C" int tp_pint_matvec(const void* d_ops, const void* d_C, int64_t c_pstride, const void* d_x, const void* d_b,
+                              void* d_y, int N, int L, int P, int fp64, void* stream) {
+  if (!d_ops || !d_C || !d_x || !d_y || N < 2 || L < 1 || P < 0 || (fp64 != 0 && fp64 != 1) ||
+      (c_pstride != 0 && c_pstride < (int64_t)L * N)) {
     set_error("tp_pint_matvec: invalid arguments (N = %d, L = %d, P = %d, fp64 = %d)", N, L, P, fp64);
     return TP_ERR_INVALID;
   }
@@ -184,10 +187,12 @@ extern "C" int tp_pint_matvec(const void* d_ops, const void* d_C, const void* d_
   const dim3 grid((unsigned)((total + 255) / 256));
   if (fp64)
     hipLaunchKernelGGL(pint_matvec_kernel<double2>, grid, dim3(256), 0, (hipStream_t)stream, (const double2*)d_ops,
-                       (const double2*)d_C, (const double2*)d_x, (const double2*)d_b, (double2*)d_y, N, L, total);
+                       (const double2*)d_C, c_pstride, (const double2*)d_x, (const double2*)d_b, (double2*)d_y, N, L,
+                       total);
   else
     hipLaunchKernelGGL(pint_matvec_kernel<float2>, grid, dim3(256), 0, (hipStream_t)stream, (const float2*)d_ops,
-                       (const float2*)d_C, (const float2*)d_x, (const float2*)d_b, (float2*)d_y, N, L, total);
+                       (const float2*)d_C, c_pstride, (const float2*)d_x, (const float2*)d_b, (float2*)d_y, N, L,
+                       total);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }

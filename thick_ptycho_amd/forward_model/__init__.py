@@ -1,6 +1,7 @@
 from .iterative_lu import PWEForwardModel, PWEIterativeLUSolver, PWEIterativeLUSolverCache
 from .ms_solver import MSForwardModelSolver
 from .operators import BoundaryType
+from .pint import PWEFullPinTSolver, PWEPetscFullPinTSolver
 
 __all__ = [
     "BoundaryType",
@@ -8,4 +9,6 @@ __all__ = [
     "PWEForwardModel",
     "PWEIterativeLUSolver",
     "PWEIterativeLUSolverCache",
+    "PWEFullPinTSolver",
+    "PWEPetscFullPinTSolver",
 ]

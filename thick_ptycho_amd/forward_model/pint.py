@@ -55,17 +55,20 @@ from .operators import BoundaryType, Tridiag, chunk_for
 
 def pint_matvec(ops: torch.Tensor, Ct: torch.Tensor, x: torch.Tensor, b: Optional[torch.Tensor] = None,
                 out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
-    """y = M x (or b - M x) for x (P, L, N); complex64 or complex128 throughout."""
+    """y = M x (or b - M x) for x (P, L, N); complex64 or complex128 throughout.
+    Ct is (L, N) for every probe or (P, L, N), one object term per probe."""
     P, L, N = x.shape
     fp64 = x.dtype == torch.complex128
     for t in (ops, Ct, x, b, out):
         if t is not None:
             assert t.is_cuda and t.dtype == x.dtype and t.is_contiguous()
-    assert ops.numel() == 6 * N and Ct.shape == (L, N) and (b is None or b.shape == x.shape)
+    assert ops.numel() == 6 * N and Ct.shape in ((L, N), (P, L, N)) and (b is None or b.shape == x.shape)
     y = out if out is not None else torch.empty_like(x)
     assert y.shape == x.shape
-    _lib.check(_lib.load().tp_pint_matvec(_lib.ptr(ops), _lib.ptr(Ct), _lib.ptr(x), _lib.ptr(b), _lib.ptr(y), N, L, P,
-                                          int(fp64), _lib.stream_handle(stream)), "tp_pint_matvec")
+    c_pstride = L * N if Ct.dim() == 3 else 0
+    _lib.check(_lib.load().tp_pint_matvec(_lib.ptr(ops), _lib.ptr(Ct), c_pstride, _lib.ptr(x), _lib.ptr(b),
+                                          _lib.ptr(y), N, L, P, int(fp64), _lib.stream_handle(stream)),
+               "tp_pint_matvec")
     return y
 
 
@@ -271,8 +274,6 @@ class PWEFullPinTSolver(BasePWESolver):
                          use_logging=use_logging, verbose=verbose, log=log, test_bcs=test_bcs, device=device)
         if simulation_space.dimension != 2:
             raise NotImplementedError("the parallel-in-time solver is built for one transverse axis")
-        if self.solve_reduced_domain:
-            raise NotImplementedError("the parallel-in-time solver does not take reduced-domain windows")
         if test_bcs is not None:
             raise NotImplementedError("test_bcs right-hand sides are not built for the parallel-in-time solver")
         if not 1 <= restart <= MAX_RESTART:
@@ -297,17 +298,19 @@ class PWEFullPinTSolver(BasePWESolver):
             self._pre[key] = PiTPreconditioner(A, B, self.block_size, self.nz - 1, self.alpha, self.device)
         return self._pre[key]
 
-    def _construct_solve_cache(self, n=None, mode: str = "forward", scan_idx: int = 0, proj_idx: int = 0) -> None:
-        t0 = time.time()
-        L = self.nz - 1
-        C = self.simulation_space.create_object_contribution(n=n, scan_index=scan_idx).reshape(-1, L)
+    def _object_rows(self, n, mode: str, scan_idx: int) -> np.ndarray:
+        """C_j as rows (L, N) for ``mode`` (full_pint.py:133-160)."""
+        C = self.simulation_space.create_object_contribution(n=n, scan_index=scan_idx).reshape(-1, self.nz - 1)
         if mode == "adjoint":
             C = np.flip(np.conj(C), axis=1)
+        return np.ascontiguousarray(C.T).astype(np.complex128)
+
+    def _construct_solve_cache(self, n=None, mode: str = "forward", scan_idx: int = 0, proj_idx: int = 0) -> None:
+        t0 = time.time()
         A, B = self._pair(mode)
-        Ct = np.ascontiguousarray(C.T)
         cache = self.projection_cache[proj_idx].modes[mode]
         cache.ops = (_ops_tensor(A, B, torch.complex64, self.device), _ops_tensor(A, B, torch.complex128, self.device))
-        Ct64 = torch.from_numpy(Ct.astype(np.complex128)).to(self.device)
+        Ct64 = torch.from_numpy(self._object_rows(n, mode, scan_idx)).to(self.device)
         cache.Ct = (Ct64.to(torch.complex64), Ct64)
         cache.pre = self.preconditioner(mode)
         cache.b = np.zeros(self.block_size, dtype=complex)
@@ -335,14 +338,16 @@ class PWEFullPinTSolver(BasePWESolver):
         b[:, 0, :-1] += up[:-1] * u[:, 1:]
         return b
 
-    def _solve_batch(self, cache, mode: str, probes: torch.Tensor, rhs_block, proj_idx: int) -> torch.Tensor:
+    def _solve_batch(self, cache, mode: str, probes: torch.Tensor, rhs_block, proj_idx: int,
+                     Ct: Optional[tuple] = None) -> torch.Tensor:
         b = self._rhs(probes, rhs_block)
+        Ct = Ct or cache.Ct
         key = (proj_idx, mode)
         x0 = self._u0_cache.get(key) if self.warm_start else None
         if x0 is not None and x0.shape != b.shape:
             x0 = None
         t0 = time.time()
-        res = gmres(cache.ops[0], cache.ops[1], cache.Ct[0], cache.Ct[1], cache.pre, b, x0=x0, atol=self.atol,
+        res = gmres(cache.ops[0], cache.ops[1], Ct[0], Ct[1], cache.pre, b, x0=x0, atol=self.atol,
                     rtol=self.rtol, restart=self.restart, maxiter=self.maxiter,
                     log=self._log if self.verbose else None)
         self.last_result = res
@@ -376,7 +381,13 @@ class PWEFullPinTSolver(BasePWESolver):
         out = torch.empty((self.num_projections, A, S, self.nz, N), dtype=torch.complex128, device=self.device)
         for p in range(self.num_projections):
             cache = self._get_or_construct_cache(n=n, mode=mode, proj_idx=p)
-            x = self._solve_batch(cache, mode, pr, rhs_block, p)
+            Ct = None
+            if self.solve_reduced_domain:  # every scan on its own window: one object term per probe
+                n_p = self.get_projected_obj(n=n, proj_idx=p)
+                rows = np.stack([self._object_rows(n_p, mode, s) for s in range(S)])
+                Ct64 = torch.from_numpy(rows).to(self.device).expand(A, S, *rows.shape[1:]).reshape(A * S, *rows.shape[1:])
+                Ct = (Ct64.to(torch.complex64).contiguous(), Ct64.contiguous())
+            x = self._solve_batch(cache, mode, pr, rhs_block, p, Ct)
             out[p, :, :, 0] = pr.reshape(A, S, N)
             out[p, :, :, 1:] = x.reshape(A, S, self.nz - 1, N)
         return out
@@ -387,3 +398,17 @@ class PWEFullPinTSolver(BasePWESolver):
         u = self.solve_device(n=n, mode=mode, rhs_block=rhs_block, probes=probes)
         u = u.transpose(-1, -2).cpu().numpy()
         return u.reshape(self.num_projections, self.num_angles, self.num_probes, *self.effective_shape)
+
+
+class PWEPetscFullPinTSolver(PWEFullPinTSolver):
+    """The PETSc form of the same solve (full_pit_petsc.py:104-327): KSP GMRES with
+    right preconditioning, the unpreconditioned residual norm and an absolute
+    tolerance only (ksp.setTolerances(atol, rtol=1e-30, max_it=20), :259-278).
+    Here rtol is 0, the first cycle runs at most 20 FP32 iterations and at most
+    two FP64-refinement cycles follow."""
+
+    def __init__(self, simulation_space, ptycho_probes, bc_type=BoundaryType.IMPEDANCE, results_dir="",
+                 use_logging=False, verbose=False, log=None, alpha=1e-2, atol=1e-8, test_bcs=None, device=None):
+        super().__init__(simulation_space, ptycho_probes, bc_type=bc_type, results_dir=results_dir,
+                         use_logging=use_logging, verbose=verbose, log=log, alpha=alpha, atol=atol,
+                         test_bcs=test_bcs, device=device, rtol=0.0, restart=20, maxiter=3)
