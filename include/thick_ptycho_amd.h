@@ -57,6 +57,9 @@
  *                      (learned-preconditioned) alternative to the per-slice
  *                      LU solve of iterative_lu.py:185-193 in 3D, with the
  *                      banded factor above as its fallback.
+ *   tp_pie_adjoint_transmission, tp_pie_update (with tp_ms_sweep, conj_h)
+ *                   <- ReconstructorMS._apply_backprop_updates / _update_object_n
+ *                      (reconstruction/ms_reconstructor.py:109-190), 3PIE.
  *   tp_pint_dots, tp_pint_combine
  *                   <- the Arnoldi projections inside scipy's gmres as called
  *                      at full_pint.py:240-248.
@@ -430,6 +433,23 @@ int tp_pint_matvec(const void* d_ops, const void* d_C, int64_t c_pstride, const 
  *   sum_e |w[p, e]|^2 (real part) in slot nrows; the caller sums over b.
  * tp_pint_combine writes out[p, e] = w[p, e] - sum_{i < nrows} V[p, i, e] c[p, i]
  *   (w null: 0; w may alias out). */
+/* ---------------------------------------------------------------- multislice 3PIE
+ * ReconstructorMS._apply_backprop_updates / _update_object_n
+ * (reconstruction/ms_reconstructor.py:109-190) for one scan position, split in
+ * three launches on the unchanged object: tp_pie_adjoint_transmission, a
+ * tp_ms_sweep with conj_h = 1 over the nz factors it writes (starting from the
+ * corrected exit wave), and tp_pie_update.
+ * n: complex128 rows of the scan's window (N rows x nz, row stride ld >= nz).
+ * tp_pie_adjoint_transmission: Tadj (nz, N) complex64, row 0 = 1,
+ *   row j = conj(exp(i k (n[:, nz-j] - n_medium) dz)).
+ * tp_pie_update: psi (nz, N) forward planes, back (nz + 1, N) the sweep's planes;
+ *   for every slice z: x = conj(T_z) back[nz - z],
+ *   n[:, z] += step conj(psi_z) (x - psi_z) / (max |psi_z|^2 + eps) / (i k dz). */
+int tp_pie_adjoint_transmission(const void* d_n, int64_t ld, int N, int nz, double k, double dz, double n_medium,
+                                void* d_Tadj, void* stream);
+int tp_pie_update(void* d_n, int64_t ld, int N, int nz, const void* d_psi, const void* d_back, double k, double dz,
+                  double n_medium, double step, double eps, void* stream);
+
 int tp_pint_dots(const void* d_V, int64_t v_pstride, int64_t v_rstride, int nrows, const void* d_w,
                  int64_t w_pstride, int64_t len, int P, int nblk, void* d_partial, void* stream);
 int tp_pint_combine(const void* d_V, int64_t v_pstride, int64_t v_rstride, int nrows, const void* d_c, int c_pstride,

@@ -1,0 +1,59 @@
+"""Multislice 3PIE on the GPU against the FP64 oracle's restatement of
+ReconstructorMS.reconstruct (same probes, data and steps): loss history and
+object after each of a few epochs, full domain and reduced-domain windows, and
+the exit-wave (no phase retrieval) form.  Tolerances relative L2, per test."""
+
+import numpy as np
+import pytest
+
+from oracle import multislice as oms
+from oracle import pie
+from tests.cases import rel_l2
+from thick_ptycho_amd import SimulationSpace, reconstruct, workloads
+
+pytestmark = pytest.mark.gpu
+K, DX, DZ = 2 * np.pi, 0.5, 1.0
+
+
+def _case(nx=128, nz=16, S=9, seed=3):
+    rng = np.random.default_rng(seed)
+    n_true = workloads.blob_object_2d(nx, nz, 5, rng, sigma=(3.0, 10.0), delta=(0, 2e-2), beta=(0, 2e-3))
+    probes = workloads.gaussian_probes_1d(nx, np.linspace(16, nx - 16, S), 10.0)
+    ex = oms.solve_batched(n_true, probes, DX, K, DZ, keep="exit")
+    return n_true, probes, ex
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+def test_3pie_matches_oracle(gpu, iters):
+    nx, nz = 128, 16
+    n_true, probes, ex = _case(nx, nz)
+    I = np.abs(np.fft.fft(ex, axis=-1)) ** 2
+    ss = SimulationSpace(nx, nz, DX, DZ, wavelength=2 * np.pi / K, num_probes=probes.shape[0])
+    rc = reconstruct.ReconstructorMS(ss, I, probes)
+    n_gpu, h_gpu = rc.reconstruct(max_iters=iters, alpha_obj=0.1)
+    n_ref, h_ref = pie.reconstruct_3pie(probes, I, nx, nz, DX, K, DZ, max_iters=iters, alpha_obj=0.1)
+    d = rel_l2(n_gpu - 1.0, n_ref - 1.0)
+    print(f"{iters} epochs: loss {h_gpu} vs {h_ref}; object update rel-L2 {d:.2e}")
+    assert np.allclose(h_gpu, h_ref, rtol=1e-4)
+    assert d < 1e-4
+
+
+def test_3pie_exit_waves_and_reduced_windows(gpu):
+    # window width step + pad - 1 = 128 (the multislice kernels take multiples of 64)
+    nx, nz, S, step = 369, 12, 11, 24
+    windows = SimulationSpace.reference_scan_windows(nx, S, step, nx - S * step)
+    ss = SimulationSpace(nx, nz, DX, DZ, wavelength=2 * np.pi / K, num_probes=S, reduced_windows=windows)
+    W = ss.effective_nx
+    assert W == 128
+    rng = np.random.default_rng(4)
+    n_true = workloads.blob_object_2d(nx, nz, 8, rng, sigma=(3.0, 10.0), delta=(0, 2e-2), beta=(0, 2e-3))
+    probes = workloads.gaussian_probes_1d(W, [W / 2] * S, 12.0)
+    ex = np.stack([oms.solve_single_probe(n_true[a:b], probes[i], DX, K, DZ)[:, -1] for i, (a, b) in enumerate(windows)])
+    rc = reconstruct.ReconstructorMS(ss, ex, probes, phase_retrieval=False)
+    n_gpu, h_gpu = rc.reconstruct(max_iters=2, alpha_obj=0.1)
+    n_ref, h_ref = pie.reconstruct_3pie(probes, ex, nx, nz, DX, K, DZ, max_iters=2, alpha_obj=0.1, windows=windows,
+                                        phase_retrieval=False)
+    d = rel_l2(n_gpu - 1.0, n_ref - 1.0)
+    print(f"reduced windows, exit-wave data: loss {h_gpu} vs {h_ref}; object update rel-L2 {d:.2e}")
+    assert np.allclose(h_gpu, h_ref, rtol=1e-4)
+    assert d < 1e-4

@@ -1,4 +1,5 @@
-"""One-axis reconstruction driver and the reference's data files.
+"""One-axis reconstruction drivers (NLCG / steepest descent on the CN gradient,
+multislice 3PIE) and the reference's data files.
 
 Data files (numpy ``.npz``, the keys the reference writes):
 
@@ -27,6 +28,7 @@ from typing import Dict, List, Optional
 import numpy as np
 import torch
 
+from . import _lib
 from . import gradient as tg
 
 
@@ -122,3 +124,92 @@ class Reconstruction2D:
         n_est = n.cpu().numpy()
         return {"n_est": n_est, "wave_field": self.wave_field(n) if wave_field else None,
                 "residual_history": hist, "alpha": alpha}
+
+
+class ReconstructorMS:
+    """Multislice 3PIE reconstruction on the GPU (ms_reconstructor.py:7-190).
+
+    Scan positions are visited in the reference's (projection, angle, scan)
+    order and each updates the object before the next is propagated.  Per scan:
+    ``tp_ms_transmission`` and a ``tp_ms_sweep`` keeping every plane (forward),
+    ``tp_amplitude_seed`` with scale 1/N for the modulus constraint
+    (psi_corr = psi_L - seed), ``tp_pie_adjoint_transmission`` and a conj(H)
+    ``tp_ms_sweep`` for the back-propagation, ``tp_pie_update`` for every
+    slice.  ``probes`` (A, S, W) or (S, W) are given explicitly (the reference
+    builds them from its simulation config); ``data`` (A*S, W) are intensities
+    or, with ``phase_retrieval=False``, exit waves.  Reduced-domain spaces update
+    each scan's window of the object."""
+
+    def __init__(self, simulation_space, data, probes, phase_retrieval: bool = True, verbose: bool = False,
+                 device=None):
+        from .forward_model import ms_engine
+
+        self._ms = ms_engine
+        ss = simulation_space
+        if ss.dimension != 2:
+            raise NotImplementedError("3PIE runs on one transverse axis (the reference's multislice)")
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.ss, self.phase_retrieval, self.verbose = ss, phase_retrieval, verbose
+        self.k, self.nz, self.dz, self.dx = ss.k, ss.nz, ss.dz, ss.dx
+        n_med = complex(getattr(ss, "n_medium", 1.0))
+        if n_med.imag != 0:
+            raise ValueError("n_medium must be real for the multislice kernels")
+        self.n_medium = n_med.real
+        self.nx = ss.nx
+        self.W = ss.effective_nx if ss.solve_reduced_domain else ss.nx
+        pr = np.asarray(probes).reshape(-1, self.W)
+        self.num_scans = ss.num_probes
+        reps = ss.num_projections  # the reference visits every projection with the same object and probes
+        self.probes = torch.from_numpy(np.tile(pr, (reps, 1))).to(self.dev, torch.complex64)
+        d = np.asarray(data).reshape(self.probes.shape[0], self.W)
+        if phase_retrieval:
+            self.target = torch.from_numpy(np.sqrt(d.real.astype(float))).to(self.dev, torch.float32)
+        else:
+            self.target = torch.from_numpy(d.astype(complex)).to(self.dev, torch.complex64)
+        self.windows = None
+        if ss.solve_reduced_domain:
+            self.windows = list(getattr(ss, "_scan_windows"))
+
+    def _window(self, idx: int):
+        if self.windows is None:
+            return 0, self.nx
+        return self.windows[idx % self.num_scans]
+
+    def reconstruct(self, max_iters: int = 10, alpha_obj: float = 1.0, refractive_index=None):
+        """(n_est complex128 (nx, nz), loss history: sqrt(mean |psi_corr - psi_L|^2) per epoch)."""
+        ms = self._ms
+        nz, W = self.nz, self.W
+        if refractive_index is None:
+            n = torch.full((self.nx, nz), self.n_medium, dtype=torch.complex128, device=self.dev)
+        else:
+            n = torch.as_tensor(np.asarray(refractive_index)).to(self.dev, torch.complex128).clone()
+        H = ms.propagator(W, self.dx, self.k, self.dz, self.dev)
+        psi = torch.empty((1, nz, W), dtype=torch.complex64, device=self.dev)
+        back = torch.empty((1, nz + 1, W), dtype=torch.complex64, device=self.dev)
+        Tadj = torch.empty((nz, W), dtype=torch.complex64, device=self.dev)
+        lib = _lib.load()
+        hist = []
+        for it in range(max_iters):
+            err2 = torch.zeros((), dtype=torch.float64, device=self.dev)
+            for idx in range(self.probes.shape[0]):
+                a, b = self._window(idx)
+                nw = n[a:b]
+                T = ms.transmission(nw, self.k, self.dz, self.n_medium)
+                ms.sweep([ms.MsScanSpec(T, H, self.probes[idx:idx + 1], out=psi)])
+                ex = psi[:, nz - 1].contiguous()
+                if self.phase_retrieval:
+                    seed, _ = tg.amplitude_seed(ex, self.target[idx:idx + 1], 1.0 / W)
+                    corr = ex - seed
+                else:
+                    corr = self.target[idx:idx + 1].contiguous()
+                err2 += ((corr - ex).abs() ** 2).sum().double()
+                sh = _lib.stream_handle()
+                _lib.check(lib.tp_pie_adjoint_transmission(_lib.ptr(nw), nz, W, nz, float(self.k), float(self.dz),
+                                                           self.n_medium, _lib.ptr(Tadj), sh),
+                           "tp_pie_adjoint_transmission")
+                ms.sweep([ms.MsScanSpec(Tadj, H, corr, out=back, conj_h=True)])
+                _lib.check(lib.tp_pie_update(_lib.ptr(nw), nz, W, nz, _lib.ptr(psi), _lib.ptr(back), float(self.k),
+                                             float(self.dz), self.n_medium, alpha_obj / (it + 1), 1e-12, sh),
+                           "tp_pie_update")
+            hist.append(float(torch.sqrt(err2 / (self.probes.shape[0] * W))))
+        return n.cpu().numpy(), hist
