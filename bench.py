@@ -11,9 +11,12 @@ probe·slice steps).  Inputs (object coefficients, probes) are resident in HBM
 before the timed region.
 
 Also reported beside the headline: the plain form (one launch per slice), the
-multislice propagator on the main scan (1024-point FFT per slice), the
-roofline of the dominant kernel (the persistent sweep) and the FP64 CPU
-oracle timed on a bounded sample on this host.
+multislice propagator on the main scan (1024-point FFT per slice) and its
+gradient, reduced-domain windows, the parallel-in-time GMRES solver, the 3PIE
+and NLCG reconstruction drivers, the two-transverse-axis banded LU / learned
+Krylov paths and reconstruction loop, the roofline of the dominant kernel
+(the persistent sweep) and the FP64 CPU oracle timed on a bounded sample on
+this host.
 """
 
 from __future__ import annotations
@@ -280,6 +283,49 @@ def main() -> int:
             log(f"PiT {nx_p}x{nz_p}x{P_p}: {t_p * 1e3:.2f} ms per solve, {res.iterations} iterations, "
                 f"{extras['pint'][-1]['value']:.4g} steps/s")
             del sol_p
+
+        # reconstruction drivers: multislice 3PIE (scan after scan) and NLCG on the CN gradient
+        from thick_ptycho_amd import reconstruct
+        from thick_ptycho_amd.forward_model import ms_engine
+
+        nx_e, nz_e, S_e = 256, 256, 16
+        ss_e = SimulationSpace(nx_e, nz_e, 0.5, 1.0, num_probes=S_e)
+        n_e = workloads.blob_object_2d(nx_e, nz_e, 12, np.random.default_rng(61), delta=(0, 2e-2), beta=(0, 2e-3))
+        pr_e = workloads.gaussian_probes_1d(nx_e, np.linspace(24, nx_e - 24, S_e), 16.0)
+        ex_e = torch.empty((S_e, nx_e), dtype=torch.complex64, device=dev)
+        ms_engine.sweep([ms_engine.MsScanSpec(
+            ms_engine.transmission(torch.from_numpy(n_e).to(dev), ss_e.k, 1.0, 1.0),
+            ms_engine.propagator(nx_e, 0.5, ss_e.k, 1.0, dev), torch.from_numpy(pr_e).to(dev, torch.complex64),
+            exit=ex_e)])
+        I_e = np.abs(np.fft.fft(ex_e.cpu().numpy(), axis=-1)) ** 2
+        rc_e = reconstruct.ReconstructorMS(ss_e, I_e, pr_e)
+        rc_e.reconstruct(max_iters=1, alpha_obj=0.1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, h_e = rc_e.reconstruct(max_iters=3, alpha_obj=0.1)
+        t_e = (time.perf_counter() - t0) / 3
+        extras["pie_3pie"] = {
+            "workload": f"multislice 3PIE, {nx_e}x{nz_e}, {S_e} scans, alpha 0.1, phase retrieval",
+            "ms_per_epoch": t_e * 1e3,
+            "value": 2 * S_e * nz_e / t_e,
+            "loss": h_e,
+        }
+        log(f"3PIE: {t_e * 1e3:.2f} ms per epoch ({extras['pie_3pie']['value']:.4g} slice steps/s)")
+
+        sc2 = workloads.config2().scans[0]
+        Ig = np.abs(np.fft.fft(sc2.probes, axis=-1)) ** 2  # synthetic data: the probes' free-space far field
+        rc2 = reconstruct.Reconstruction2D(Ig, sc2.probes, sc2.k, sc2.dx, sc2.dz, sc2.bc)
+        rc2.run(sc2.n, 1, method="nlcg", alpha=1e-9, wave_field=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc2.run(sc2.n, 3, method="nlcg", alpha=1e-9, wave_field=False)
+        t_g2 = (time.perf_counter() - t0) / 4  # 3 iterations + the initial gradient
+        extras["nlcg_2d"] = {
+            "workload": f"NLCG on the CN amplitude gradient, {sc2.nx}x{sc2.nz}, {sc2.probes.shape[0]} probes",
+            "ms_per_gradient": t_g2 * 1e3,
+            "value": 2 * sc2.probes.shape[0] * (sc2.nz - 1) / t_g2,
+        }
+        log(f"NLCG 2D: {t_g2 * 1e3:.2f} ms per gradient ({extras['nlcg_2d']['value']:.4g} steps/s fwd+adj)")
 
     cnn = None
     if not args.no_3d:
