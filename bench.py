@@ -78,7 +78,8 @@ def main() -> int:
     cpu_results = None
     if not args.no_cpu:  # host-only work, before this process touches the GPU (the pool forks)
         cpu_results = (cpu_baseline(wl.scans[0], args.cpu_probes),
-                       cpu_baseline_parallel(wl.scans[0], args.cpu_probes_per_proc))
+                       cpu_baseline_parallel(wl.scans[0], args.cpu_probes_per_proc),
+                       cpu_baseline_widened())
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.copy_calibration:
@@ -290,7 +291,7 @@ def main() -> int:
 
         nx_e, nz_e, S_e = 256, 256, 16
         ss_e = SimulationSpace(nx_e, nz_e, 0.5, 1.0, num_probes=S_e)
-        n_e = workloads.blob_object_2d(nx_e, nz_e, 12, np.random.default_rng(61), delta=(0, 2e-2), beta=(0, 2e-3))
+        n_e = workloads.blob_object_2d(nx_e, nz_e, 12, np.random.default_rng(61), delta=(0, 2e-3), beta=(0, 2e-4))
         pr_e = workloads.gaussian_probes_1d(nx_e, np.linspace(24, nx_e - 24, S_e), 16.0)
         ex_e = torch.empty((S_e, nx_e), dtype=torch.complex64, device=dev)
         ms_engine.sweep([ms_engine.MsScanSpec(
@@ -299,13 +300,13 @@ def main() -> int:
             exit=ex_e)])
         I_e = np.abs(np.fft.fft(ex_e.cpu().numpy(), axis=-1)) ** 2
         rc_e = reconstruct.ReconstructorMS(ss_e, I_e, pr_e)
-        rc_e.reconstruct(max_iters=1, alpha_obj=0.1)
+        rc_e.reconstruct(max_iters=1, alpha_obj=1e-3)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, h_e = rc_e.reconstruct(max_iters=3, alpha_obj=0.1)
+        _, h_e = rc_e.reconstruct(max_iters=3, alpha_obj=1e-3)
         t_e = (time.perf_counter() - t0) / 3
         extras["pie_3pie"] = {
-            "workload": f"multislice 3PIE, {nx_e}x{nz_e}, {S_e} scans, alpha 0.1, phase retrieval",
+            "workload": f"multislice 3PIE, {nx_e}x{nz_e}, {S_e} scans, alpha 1e-3, phase retrieval",
             "ms_per_epoch": t_e * 1e3,
             "value": 2 * S_e * nz_e / t_e,
             "loss": h_e,
@@ -335,7 +336,13 @@ def main() -> int:
 
     cpu, cpu_par = None, None
     if not args.no_cpu:
-        cpu, cpu_par = cpu_results
+        cpu, cpu_par, cpu_wide = cpu_results
+        if "pint" in extras:
+            extras["pint"][0]["cpu_oracle"] = cpu_wide["pint_256"]
+            log(f"cpu PiT oracle: {cpu_wide['pint_256']['value']:.4g} steps/s")
+        if "pie_3pie" in extras:
+            extras["pie_3pie"]["cpu_oracle"] = cpu_wide["pie_256"]
+            log(f"cpu 3PIE oracle: {cpu_wide['pie_256']['ms_per_epoch']:.1f} ms per epoch")
         log(f"cpu baseline: {cpu['value']:.4g} steps/s ({cpu['sample']})")
         log(f"cpu baseline, probe-parallel: {cpu_par['value']:.4g} steps/s ({cpu_par['sample']})")
 
@@ -619,6 +626,43 @@ def cpu_baseline(sc, probes: int) -> dict:
                   f"factorisation, complex128, 1 thread ({cpu_model}; host has {os.cpu_count()} cpus)",
         "seconds": dt,
     }
+
+
+def cpu_baseline_widened() -> dict:
+    """FP64 oracle, 1 thread, on the workloads of the widened lines: the
+    parallel-in-time solve (scipy GMRES with the splu-factored alpha-circulant
+    blocks, one probe at a time as the reference) on 4 of the 16 probes of the
+    256 x 256 case, and one 3PIE epoch of the 256 x 256, 16-scan case."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import multislice as oms
+    from oracle import pie, pint, pwe
+    from thick_ptycho_amd import workloads
+
+    out = {}
+    with threadpool_limits(1):
+        nx, nz, P = 256, 256, 16
+        g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=2 * np.pi)
+        n = workloads.blob_object_2d(nx, nz, 12, np.random.default_rng(60))
+        pr = workloads.gaussian_probes_1d(nx, np.linspace(nx / 4, 3 * nx / 4, P), nx / 16)
+        t0 = time.perf_counter()
+        for p in range(4):
+            pint.pit_solve(g, "impedance", n, pr[p])
+        dt = (time.perf_counter() - t0) / 4
+        out["pint_256"] = {"value": (nz - 1) / dt, "unit": "probe·slice steps/s", "cores": 1, "kind": "port",
+                           "sample": "4 probes of the 256 x 256 PiT case, alpha 1e-2, rtol 1e-5, complex128"}
+        nx_e, nz_e, S_e = 256, 256, 16
+        n_e = workloads.blob_object_2d(nx_e, nz_e, 12, np.random.default_rng(61), delta=(0, 2e-3), beta=(0, 2e-4))
+        pr_e = workloads.gaussian_probes_1d(nx_e, np.linspace(24, nx_e - 24, S_e), 16.0)
+        ex = oms.solve_batched(n_e, pr_e, 0.5, 2 * np.pi, 1.0, keep="exit")
+        I_e = np.abs(np.fft.fft(ex, axis=-1)) ** 2
+        t0 = time.perf_counter()
+        pie.reconstruct_3pie(pr_e, I_e, nx_e, nz_e, 0.5, 2 * np.pi, 1.0, max_iters=1, alpha_obj=1e-3)
+        dt = time.perf_counter() - t0
+        out["pie_256"] = {"value": 2 * S_e * nz_e / dt, "unit": "slice steps/s", "ms_per_epoch": dt * 1e3,
+                          "cores": 1, "kind": "port", "sample": "one 3PIE epoch, 256 x 256, 16 scans, complex128"}
+    return out
 
 
 _CPU_SCAN = None
