@@ -15,3 +15,12 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_widened_cpu_leg():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    r = bench.cpu_baseline_widened()
+    for key in ("pint_256", "pie_256"):
+        assert r[key]["value"] > 0 and r[key]["cores"] == 1 and r[key]["kind"] == "port"
