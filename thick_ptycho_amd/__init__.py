@@ -2,11 +2,17 @@
 
 Drop-in solver classes (same constructors and methods as bryceshirley/thick_ptycho):
 ``PWEIterativeLUSolver`` (Crank-Nicolson slice stepping with per-slice LU,
-forward / adjoint / reverse) and ``MSForwardModelSolver`` (angular-spectrum
-multislice), both running hand-written HIP kernels from ``libtp_hip.so``.
+forward / adjoint / reverse), ``PWEFullPinTSolver`` / ``PWEPetscFullPinTSolver``
+(the whole space-z system by alpha-circulant preconditioned GMRES) and
+``MSForwardModelSolver`` (angular-spectrum multislice), all running
+hand-written HIP kernels from ``libtp_hip.so``.  Reconstruction drivers live in
+``thick_ptycho_amd.reconstruct`` (NLCG, 3PIE, data files) and
+``thick_ptycho_amd.recon3d`` (two transverse axes).
 """
 
-from .forward_model import BoundaryType, MSForwardModelSolver, PWEIterativeLUSolver
+from .forward_model import (BoundaryType, MSForwardModelSolver, PWEFullPinTSolver, PWEIterativeLUSolver,
+                            PWEPetscFullPinTSolver)
 from .simulation import SimulationSpace
 
-__all__ = ["BoundaryType", "MSForwardModelSolver", "PWEIterativeLUSolver", "SimulationSpace"]
+__all__ = ["BoundaryType", "MSForwardModelSolver", "PWEFullPinTSolver", "PWEIterativeLUSolver",
+           "PWEPetscFullPinTSolver", "SimulationSpace"]
