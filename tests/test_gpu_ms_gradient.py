@@ -65,3 +65,24 @@ def test_ms_gradient_two_axes(gpu):
     # 8192 detector bins per probe, many where |fft2 psi_L| is ~1e-6 of its peak:
     # the seed's phase there follows complex64 rounding of the exit waves
     assert rel_l2(got, ref) < 5e-3
+
+
+def test_ms_autograd_backward(gpu):
+    """MSExitWaves under torch.autograd against the FP64 multislice gradient."""
+    import torch
+
+    rng = np.random.default_rng(31)
+    nx, nz, P, dx, dz = 256, 20, 4, 2.0, 4.0
+    n_true = workloads.blob_object_2d(nx, nz, 5, rng)
+    n0 = workloads.blob_object_2d(nx, nz, 5, np.random.default_rng(32))
+    probes = workloads.gaussian_probes_1d(nx, np.linspace(60, 200, P), 25.0)
+    I = np.abs(np.fft.fft(oms.solve_batched(n_true, probes, dx, K, dz, keep="exit"), axis=-1)) ** 2
+    ref, _ = oms.ms_gradient(n0, probes, dx, K, dz, I)
+    n = torch.tensor(n0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    ex = gradient.MSExitWaves.apply(n, torch.tensor(probes, device="cuda"), dx, K, dz)
+    a = torch.tensor(np.sqrt(I), dtype=torch.float32, device="cuda")
+    J = ((torch.fft.fft(ex, dim=-1).abs() - a) ** 2).sum()
+    J.backward()
+    err = rel_l2(n.grad.cpu().numpy(), ref)
+    print(f"autograd multislice gradient rel-L2 {err:.2e}")
+    assert err < 1e-3  # measured 1.3e-5

@@ -223,6 +223,15 @@ def _ms_gradient_core(T, u0, run, seed_fn, k: float, dz: float, seed=None):
     sd, loss = seed_fn(ex)
     if seed is not None:
         sd = _dev_tensor(seed, torch.complex64, dev).reshape(P, N).contiguous()
+    return float(loss.double().sum()), _ms_backward(T, u, sd, run, k, dz)
+
+
+def _ms_backward(T, u, sd, run, k: float, dz: float):
+    """dJ/dRe n + i dJ/dIm n, complex128 (N, L + 1), from all forward planes u
+    (P, L + 1, N) and the seed sd = dJ/d conj(psi_L) (P, N)."""
+    L, N = T.shape
+    P = u.shape[0]
+    dev = T.device
     Tadj = torch.empty_like(T)
     Tadj[0] = 1
     if L > 1:
@@ -234,7 +243,42 @@ def _ms_gradient_core(T, u0, run, seed_fn, k: float, dz: float, seed=None):
                                              N, _lib.ptr(G), _lib.stream_handle()), "tp_plane_conj_dot")
     grad = torch.zeros((N, L + 1), dtype=torch.complex128, device=dev)
     grad[:, :L] = ((-2j * k * dz) * (T.conj() * G)).transpose(0, 1).to(torch.complex128)
-    return float(loss.double().sum()), grad
+    return grad
+
+
+class MSExitWaves(torch.autograd.Function):
+    """n (complex128 (nx, nz), CUDA) -> exit planes (complex64 (P, nx)) of the
+    multislice (ms_solver.py:90-159); backward runs the conj(H) adjoint sweep
+    from the incoming gradient (torch's convention: grad_exit = 2 dJ/d conj(exit))
+    and returns dJ/dRe n + i dJ/dIm n.  Every forward plane is kept for the
+    backward pass (P nz nx complex64)."""
+
+    @staticmethod
+    def forward(ctx, n, probes, dx: float, k: float, dz: float, n_medium: float = 1.0):
+        from .forward_model import ms_engine
+
+        nt = n.detach().to(torch.complex128).contiguous()
+        nx, nz = nt.shape
+        T = ms_engine.transmission(nt, k, dz, n_medium)
+        H = ms_engine.propagator(nx, dx, k, dz, nt.device)
+        u0 = probes.detach().to(nt.device, torch.complex64).reshape(-1, nx).contiguous()
+        u = torch.empty((u0.shape[0], nz, nx), dtype=torch.complex64, device=nt.device)
+        ms_engine.sweep([ms_engine.MsScanSpec(T, H, u0, out=u)])
+        ctx.save_for_backward(T, H, u)
+        ctx.k, ctx.dz = k, dz
+        return u[:, nz - 1].contiguous()
+
+    @staticmethod
+    def backward(ctx, grad_exit):
+        from .forward_model import ms_engine
+
+        T, H, u = ctx.saved_tensors
+
+        def run(Tz, x0, out, conj_h):
+            ms_engine.sweep([ms_engine.MsScanSpec(Tz, H, x0, out=out, conj_h=conj_h)])
+
+        sd = (0.5 * grad_exit).to(torch.complex64).contiguous()
+        return _ms_backward(T, u, sd, run, ctx.k, ctx.dz), None, None, None, None, None
 
 
 def ms_amplitude_gradient(n, probes, sqrt_i, dx: float, k: float, dz: float, n_medium: float = 1.0, seed=None):
