@@ -24,7 +24,10 @@ Pinning (how each function is checked independently of its own formula):
   * multislice: plane-wave eigenvectors with closed-form eigenvalues, constant
     object phase, Parseval.
   * far field: ``numpy.fft`` identities (Parseval, single-mode delta).
-  * reference gradient (``compute_grad_least_squares``): parity unpinned
-    beyond restatement consistency checks (see tests/test_oracle_gradient.py);
-    the exact gradient is pinned by finite differences.
+  * reference gradient (``compute_grad_least_squares``): no golden values;
+    pinned where the object is uniform along z, where its adjoint coincides
+    with the exact one and it must equal a closed form in the finite-difference
+    pinned exact adjoint states (tests/test_oracle_gradient.py); the z-varying
+    case rests on the restatement.  The exact gradient is pinned by finite
+    differences.
 """

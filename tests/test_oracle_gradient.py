@@ -4,9 +4,11 @@
 (h = 1e-6 in FP64) on random entries of n, real and imaginary parts.  The
 reference-form gradient (``reference_gradient``) has no independent golden
 values -- the reference suite never tests it and importing the reference was
-not used -- so its parity is unpinned beyond the structural checks below:
-its adjoint states equal the restated reference adjoint sweep, and its error
-term is the exact loss seed scaled by 1/N (base_reconstructor.py:171-207).
+not used.  It is pinned where the object is uniform along z (its adjoint then
+coincides with the exact one, so it equals a closed form in the exact adjoint
+states), besides the structural checks: its adjoint states equal the restated
+reference adjoint sweep, and its error term is the exact loss seed scaled by
+1/N (base_reconstructor.py:171-207).
 """
 
 import numpy as np
@@ -140,3 +142,35 @@ def test_nlcg_beta_rule(g_new, g_old, beta):
 
     got = reconstruct.beta_rule(torch.tensor(g_new, dtype=torch.float64), torch.tensor(g_old, dtype=torch.float64))
     assert got == pytest.approx(beta)
+
+
+def test_reference_gradient_equals_exact_adjoint_form_when_uniform_in_z(rng):
+    """Pin of the restated compute_grad_least_squares: when the object does not
+    vary along z the reference-form adjoint (quirk Q3: slice L+1-j's object term
+    on both sides) coincides with the exact adjoint, so the reference gradient
+    must equal -conj(eo) sum_p conj(u_s + u_{s-1}) lambda_s / N with lambda from
+    the exact adjoint (the path pinned by finite differences above), eo =
+    (dz/2)(k/i), the u_0 term absent at s = 1 and the 1/N of its exit-wave
+    error.  A wrong flip, sign, conjugate or seed scale fails it; with an object
+    varying along z the two differ (the quirk itself)."""
+    nx, nz, P = 40, 9, 3
+    g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=2 * np.pi)
+    prof = 1 - (5e-3 + 5e-5j) * np.exp(-((np.arange(nx) - 20) / 6.0) ** 2)
+    probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+    I = np.abs(rng.standard_normal((P, nx))) ** 2 * nx
+    eo = (g.dz / 2) * (g.k / 1j)
+
+    def both(n):
+        gr, gi = og.reference_gradient(g, "impedance", n, probes, I)
+        ref = (gr + 1j * gi).reshape(nz - 1, nx).T
+        u = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", n, "forward"), probes, "forward")
+        lam = og.exact_adjoint_states(g, "impedance", n, og.amplitude_loss_seed(u[:, :, -1], I))
+        pred = np.stack([-np.conj(eo) * np.sum(np.conj(u[:, :, s] + (u[:, :, s - 1] if s > 1 else 0))
+                                                * lam[:, :, s], axis=0) / nx for s in range(1, nz)], axis=1)
+        return ref, pred
+
+    ref, pred = both(np.repeat(prof[:, None], nz, axis=1))
+    assert np.linalg.norm(ref - pred) <= 1e-12 * np.linalg.norm(pred)
+    zvar = np.repeat(prof[:, None], nz, axis=1) * (1 + 2e-3 * np.arange(nz))[None, :]
+    ref, pred = both(zvar)
+    assert np.linalg.norm(ref - pred) > 1e-6 * np.linalg.norm(pred)
