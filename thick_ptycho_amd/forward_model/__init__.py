@@ -1,3 +1,4 @@
+from .full_lu import PWEFullLUSolver
 from .iterative_lu import PWEForwardModel, PWEIterativeLUSolver, PWEIterativeLUSolverCache
 from .ms_solver import MSForwardModelSolver
 from .operators import BoundaryType
@@ -9,6 +10,7 @@ __all__ = [
     "PWEForwardModel",
     "PWEIterativeLUSolver",
     "PWEIterativeLUSolverCache",
+    "PWEFullLUSolver",
     "PWEFullPinTSolver",
     "PWEPetscFullPinTSolver",
 ]
