@@ -59,3 +59,22 @@ def test_nlcg_decreases_loss_and_files_round_trip(gpu, tmp_path):
     back = reconstruct.load_reconstruction(path)
     assert np.array_equal(back["n_est"], out["n_est"]) and np.array_equal(back["residual_history"], h)
     assert np.array_equal(back["wave_field"], wf)
+
+
+def test_reference_named_entry_point(gpu):
+    """ReconstructorPWE: (n_est, wave_field, residual) with the exit-wave RMSE
+    residual of the reference, equal to the oracle's RMSE of the same errors, and
+    the tolerance stop."""
+    from thick_ptycho_amd import SimulationSpace
+
+    g, n_true, n0, probes, I = _problem(seed=5)
+    ss = SimulationSpace(256, 32, 2.0, 4.0, wavelength=1.0, num_probes=probes.shape[0])
+    rc = reconstruct.ReconstructorPWE(ss, I, probes, bc_type="dirichlet")
+    n_est, wf, res = rc.reconstruct(max_iters=3, n_initial=n0)
+    assert n_est.shape == (256, 32) and wf.shape == (1, probes.shape[0], 256, 31) and 2 <= len(res) <= 4
+    u = pwe.solve_batched(g, pwe.build_lu_cache(g, "dirichlet", n0, "forward"), probes, keep="exit")
+    e = u - og.phase_retrieval_model(u, I)
+    assert res[0] == pytest.approx(np.sqrt(np.mean(np.abs(e) ** 2)), rel=1e-4)
+    assert res[-1] < res[0]
+    _, _, res2 = rc.reconstruct(max_iters=3, n_initial=n0, tol=10 * res[0])
+    assert len(res2) == 1

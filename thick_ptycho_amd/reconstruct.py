@@ -92,12 +92,19 @@ class Reconstruction2D:
         _, g1 = self.gradient(n - d)
         return float(torch.linalg.vector_norm(g1 - g0) / torch.linalg.vector_norm(d))
 
+    def rmse(self, J: float) -> float:
+        """sqrt(mean |e|^2) of the reference's exit-wave error e = u - ifft(sqrt(I) U/|U|)
+        (pwe_reconstructor.py:298-301): ||e_p||^2 = J_p / N by Parseval."""
+        P, N = self.sqrt_i.shape
+        return float(np.sqrt(max(J, 0.0) / (P * N * N)))
+
     def run(self, n0, iters: int, method: str = "nlcg", alpha: Optional[float] = None,
-            wave_field: bool = True) -> dict:
+            wave_field: bool = True, tol: Optional[float] = None) -> dict:
         """Returns {'n_est': complex128 (nx, nz), 'wave_field': ``wave_field(n_est)``,
         'residual_history': [J before the first iteration and after each accepted
-        one], 'alpha': the last step}; the three arrays are what
-        ``save_reconstruction`` writes."""
+        one], 'rmse_history': the same as exit-wave RMSE, 'alpha': the last step};
+        the three arrays are what ``save_reconstruction`` writes.  With ``tol`` the
+        loop stops once the RMSE is below it (the reference's test)."""
         if method not in ("nlcg", "sd"):
             raise ValueError("method must be 'nlcg' or 'sd'")
         n = torch.as_tensor(np.asarray(n0)).to(self.dev, torch.complex128).clone()
@@ -107,6 +114,8 @@ class Reconstruction2D:
         d = -g
         hist: List[float] = [J]
         for _ in range(iters):
+            if tol is not None and self.rmse(J) < tol:
+                break
             n_new = n + alpha * d
             J_new, g_new = self.gradient(n_new)
             if J_new > J and method == "nlcg":  # safeguard: smaller step along -grad
@@ -123,7 +132,38 @@ class Reconstruction2D:
             hist.append(J)
         n_est = n.cpu().numpy()
         return {"n_est": n_est, "wave_field": self.wave_field(n) if wave_field else None,
-                "residual_history": hist, "alpha": alpha}
+                "residual_history": hist, "rmse_history": [self.rmse(v) for v in hist], "alpha": alpha}
+
+
+class ReconstructorPWE:
+    """The reference's entry point for one transverse axis (ReconstructorPWE,
+    pwe_reconstructor.py:34-368) on ``Reconstruction2D``: ``reconstruct`` returns
+    (n_est, wave_field, residual) with the reference's residual (exit-wave RMSE)
+    and stops once it is below ``tol``.  ``probes`` are given explicitly (the
+    reference builds them from its simulation config); intensity data only; the
+    step is 0.5 / Lipschitz, or ``fixed_step_size`` as a constant step; the TV /
+    low-pass object filters are not built."""
+
+    def __init__(self, simulation_space, data, probes, phase_retrieval: bool = True, solver_type: str = "iterative",
+                 bc_type="impedance", verbose: Optional[bool] = None, device=None):
+        if solver_type not in {"full", "iterative"}:
+            raise ValueError(f"Invalid solver_type: {solver_type!r}")
+        if not phase_retrieval:
+            raise NotImplementedError("exit-wave (non phase-retrieval) data are not built for this driver")
+        ss = simulation_space
+        if ss.dimension != 2:
+            raise NotImplementedError("two transverse axes: thick_ptycho_amd.recon3d.Reconstruction3D")
+        self.ss = ss
+        self.rc = Reconstruction2D(np.asarray(data).reshape(-1, ss.nx), np.asarray(probes).reshape(-1, ss.nx), ss.k,
+                                   ss.dx, ss.dz, bc_type, device=device)
+
+    def reconstruct(self, max_iters: int = 10, tol: float = 1e-8, n_initial=None, fixed_step_size=None,
+                    tv_lambda_amp: float = 0.0, low_pass_sigma_phase: float = 0.0):
+        if tv_lambda_amp or low_pass_sigma_phase:
+            raise NotImplementedError("TV / low-pass object filters are not built")
+        n0 = n_initial if n_initial is not None else np.ones(self.ss.shape, dtype=complex) * self.ss.n_medium
+        out = self.rc.run(n0, max_iters, method="nlcg", alpha=fixed_step_size, tol=tol)
+        return out["n_est"], out["wave_field"], out["rmse_history"]
 
 
 class ReconstructorMS:
