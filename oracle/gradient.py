@@ -106,15 +106,29 @@ def exact_adjoint_states(g: pwe.Grid, bc_type, n: np.ndarray, seed: np.ndarray) 
     return lam
 
 
-def exact_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_meas: np.ndarray):
-    """Exact dJ/dRe(n) + i dJ/dIm(n), shape (N, nz), for the amplitude loss."""
+def exact_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_meas: np.ndarray,
+                   exit_data: np.ndarray = None):
+    """Exact dJ/dRe(n) + i dJ/dIm(n), shape (N, nz), and J, for the amplitude loss
+    or, with ``exit_data`` (P, N), for J = sum |u_L - exit_data|^2 (seed u_L - d)."""
     N = g.block_size
     L = g.nz - 1
     P = probes.shape[0]
     fwd = pwe.build_lu_cache(g, bc_type, n, "forward")
     u = pwe.solve_batched(g, fwd, probes.reshape(P, N), "forward")
+    if exit_data is not None:
+        seed = u[:, :, -1] - exit_data
+        lam = exact_adjoint_states(g, bc_type, n, seed)
+        J = float(np.sum(np.abs(seed) ** 2))
+        return _assemble_exact(g, n, u, lam), J
     seed = amplitude_loss_seed(u[:, :, -1], I_meas)
     lam = exact_adjoint_states(g, bc_type, n, seed)
+    J = float(np.sum((np.abs(np.fft.fft(u[:, :, -1], axis=-1)) - np.sqrt(I_meas)) ** 2))
+    return _assemble_exact(g, n, u, lam), J
+
+
+def _assemble_exact(g: pwe.Grid, n: np.ndarray, u: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """2 conj(Gn) from the forward planes u and adjoint states lam (P, N, L+1)."""
+    N, L = g.block_size, g.nz - 1
     GC = np.zeros((N, L), dtype=complex)
     for s in range(1, L + 1):
         GC[:, s - 1] = np.sum(np.conj(lam[:, :, s]) * (u[:, :, s] + u[:, :, s - 1]), axis=0)
@@ -124,7 +138,7 @@ def exact_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_me
     Gn[:, :-1] += GC
     Gn[:, 1:] += GC
     Gn *= (g.dz / 4) * dcoef
-    return 2 * np.conj(Gn), float(np.sum((np.abs(np.fft.fft(u[:, :, -1], axis=-1)) - np.sqrt(I_meas)) ** 2))
+    return 2 * np.conj(Gn)
 
 
 # ----------------------------------------------------------------------------

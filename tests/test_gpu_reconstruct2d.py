@@ -78,3 +78,31 @@ def test_reference_named_entry_point(gpu):
     assert res[-1] < res[0]
     _, _, res2 = rc.reconstruct(max_iters=3, n_initial=n0, tol=10 * res[0])
     assert len(res2) == 1
+
+
+def test_exit_wave_data(gpu):
+    """phase_retrieval=False: J = sum |u_L - d|^2; steepest descent against the same
+    iterations on the FP64 exact gradient of that loss (pinned by finite
+    differences), and the reference's residual sqrt(mean |u - d|^2)."""
+    from thick_ptycho_amd import SimulationSpace
+
+    g, n_true, n0, probes, I = _problem(seed=6)
+    d = pwe.solve_batched(g, pwe.build_lu_cache(g, "dirichlet", n_true, "forward"), probes, keep="exit")
+    rc = reconstruct.Reconstruction2D(d, probes, K, 2.0, 4.0, "dirichlet", phase_retrieval=False)
+    out = rc.run(n0, 3, method="sd")
+    n = np.array(n0, dtype=complex)
+    hist = []
+    for _ in range(3):
+        grad, J = og.exact_gradient(g, "dirichlet", n, probes, None, exit_data=d)
+        hist.append(J)
+        n = n - out["alpha"] * grad.reshape(n.shape)
+    err = rel_l2(out["n_est"] - n0, n - n0)
+    print(f"exit-wave data: loss {out['residual_history']} vs {hist}; update rel-L2 {err:.2e}")
+    assert np.allclose(out["residual_history"][:3], hist, rtol=1e-4)
+    assert err < 1e-4
+    u0 = pwe.solve_batched(g, pwe.build_lu_cache(g, "dirichlet", n0, "forward"), probes, keep="exit")
+    assert out["rmse_history"][0] == pytest.approx(np.sqrt(np.mean(np.abs(u0 - d) ** 2)), rel=1e-4)
+    ss = SimulationSpace(256, 32, 2.0, 4.0, wavelength=1.0, num_probes=probes.shape[0])
+    _, _, res = reconstruct.ReconstructorPWE(ss, d, probes, phase_retrieval=False, bc_type="dirichlet").reconstruct(
+        max_iters=2, n_initial=n0)
+    assert res[-1] < res[0]

@@ -174,3 +174,28 @@ def test_reference_gradient_equals_exact_adjoint_form_when_uniform_in_z(rng):
     zvar = np.repeat(prof[:, None], nz, axis=1) * (1 + 2e-3 * np.arange(nz))[None, :]
     ref, pred = both(zvar)
     assert np.linalg.norm(ref - pred) > 1e-6 * np.linalg.norm(pred)
+
+
+def test_exit_wave_loss_gradient_finite_differences():
+    """exact_gradient with exit_data: J = sum |u_L - d|^2, central differences at
+    random entries (real and imaginary parts of n)."""
+    nx, nz, P = 24, 7, 2
+    rng = np.random.default_rng(5)
+    g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=2 * np.pi)
+    n = 1 - (rng.uniform(0, 1e-2, (nx, nz)) + 1j * rng.uniform(0, 1e-4, (nx, nz)))
+    probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+    d = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+    grad, J = og.exact_gradient(g, "impedance", n, probes, None, exit_data=d)
+
+    def loss(m):
+        u = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", m, "forward"), probes, keep="exit")
+        return float(np.sum(np.abs(u - d) ** 2))
+
+    assert J == pytest.approx(loss(n), rel=1e-12)
+    h = 1e-6
+    for i, j in [(3, 2), (12, 5), (20, 0), (7, 6)]:
+        for unit, part in ((1.0, np.real), (1j, np.imag)):
+            e = np.zeros_like(n)
+            e[i, j] = unit * h
+            fd = (loss(n + e) - loss(n - e)) / (2 * h)
+            assert fd == pytest.approx(part(grad[i, j]), rel=1e-5, abs=1e-9 * abs(J))

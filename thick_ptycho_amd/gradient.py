@@ -112,13 +112,14 @@ def _dev_tensor(x, dtype, device):
 
 
 def amplitude_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="dirichlet", checkpoint: int = 32,
-                       device=None, seed=None):
+                       device=None, seed=None, exit_data=None):
     """(loss, grad) of J(n) for one scan; grad complex128 (N, nz) on the GPU in the
     dJ/dRe(n) + i dJ/dIm(n) convention.
 
     ``seed`` (P, N), when given, replaces the loss seed dJ/d conj(u_L) that is
     otherwise computed from the exit waves and ``sqrt_i`` (the loss is still
-    evaluated from them)."""
+    evaluated from them).  ``exit_data`` (P, N) switches to the exit-wave loss
+    J = sum |u_L - exit_data|^2 (seed u_L - exit_data; ``sqrt_i`` unused)."""
     device = device or torch.device("cuda", torch.cuda.current_device())
     n_t = _dev_tensor(n, torch.complex128, device)
     N, nz = n_t.shape
@@ -126,9 +127,13 @@ def amplitude_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="di
     C = object_contribution(n_t, k, dz)
     sw = _Sweeper(C, A, B, "exact_adjoint", checkpoint)
     u0 = _dev_tensor(probes, torch.complex64, device).reshape(-1, N)
-    amp = _dev_tensor(sqrt_i, torch.float32, device).reshape(u0.shape[0], N)
     ck, ex = sw.forward(u0)
-    sd, loss = amplitude_seed(ex, amp, 1.0)
+    if exit_data is not None:
+        sd = ex - _dev_tensor(exit_data, torch.complex64, device).reshape(u0.shape[0], N)
+        loss = (sd.abs() ** 2).sum(dim=1)
+    else:
+        amp = _dev_tensor(sqrt_i, torch.float32, device).reshape(u0.shape[0], N)
+        sd, loss = amplitude_seed(ex, amp, 1.0)
     if seed is not None:
         sd = _dev_tensor(seed, torch.complex64, device).reshape(u0.shape[0], N)
     G = sw.adjoint(ck, sd)

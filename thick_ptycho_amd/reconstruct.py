@@ -63,18 +63,26 @@ def beta_rule(g_new: torch.Tensor, g_old: torch.Tensor) -> float:
 
 
 class Reconstruction2D:
-    """Reconstruct n (nx, nz) from far-field intensities I (P, nx) of probes (P, nx)."""
+    """Reconstruct n (nx, nz) from far-field intensities I (P, nx) of probes (P, nx),
+    or, with ``phase_retrieval=False``, from complex exit waves (P, nx) with the
+    loss sum |u_L - data|^2 (the reference's two data kinds, pwe_reconstructor.py)."""
 
     def __init__(self, intensity, probes, k: float, dx: float, dz: float, bc: str = "dirichlet",
-                 checkpoint: int = 32, device=None):
+                 checkpoint: int = 32, device=None, phase_retrieval: bool = True):
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
-        self.sqrt_i = torch.as_tensor(np.sqrt(np.asarray(intensity, dtype=float))).to(self.dev, torch.float32)
         self.probes = torch.as_tensor(np.asarray(probes)).to(self.dev, torch.complex64)
+        self.phase_retrieval = phase_retrieval
+        if phase_retrieval:
+            self.sqrt_i = torch.as_tensor(np.sqrt(np.asarray(intensity, dtype=float))).to(self.dev, torch.float32)
+            self.exit_data = None
+        else:
+            self.exit_data = torch.as_tensor(np.asarray(intensity)).to(self.dev, torch.complex64)
+            self.sqrt_i = torch.zeros(self.exit_data.shape, dtype=torch.float32, device=self.dev)
         self.k, self.dx, self.dz, self.bc, self.checkpoint = k, dx, dz, bc, checkpoint
 
     def gradient(self, n: torch.Tensor):
         return tg.amplitude_gradient(n, self.probes, self.sqrt_i, self.k, self.dx, self.dz, self.bc,
-                                     self.checkpoint, device=self.dev)
+                                     self.checkpoint, device=self.dev, exit_data=self.exit_data)
 
     def wave_field(self, n) -> np.ndarray:
         """Forward field of every probe at slices 1..nz-1, complex64 (1, P, nx, nz-1): the
@@ -93,10 +101,10 @@ class Reconstruction2D:
         return float(torch.linalg.vector_norm(g1 - g0) / torch.linalg.vector_norm(d))
 
     def rmse(self, J: float) -> float:
-        """sqrt(mean |e|^2) of the reference's exit-wave error e = u - ifft(sqrt(I) U/|U|)
-        (pwe_reconstructor.py:298-301): ||e_p||^2 = J_p / N by Parseval."""
+        """sqrt(mean |e|^2) of the reference's exit-wave error (pwe_reconstructor.py:298-301):
+        e = u - ifft(sqrt(I) U/|U|), ||e_p||^2 = J_p / N by Parseval, or e = u - data."""
         P, N = self.sqrt_i.shape
-        return float(np.sqrt(max(J, 0.0) / (P * N * N)))
+        return float(np.sqrt(max(J, 0.0) / (P * N * (N if self.phase_retrieval else 1))))
 
     def run(self, n0, iters: int, method: str = "nlcg", alpha: Optional[float] = None,
             wave_field: bool = True, tol: Optional[float] = None) -> dict:
@@ -140,22 +148,21 @@ class ReconstructorPWE:
     pwe_reconstructor.py:34-368) on ``Reconstruction2D``: ``reconstruct`` returns
     (n_est, wave_field, residual) with the reference's residual (exit-wave RMSE)
     and stops once it is below ``tol``.  ``probes`` are given explicitly (the
-    reference builds them from its simulation config); intensity data only; the
-    step is 0.5 / Lipschitz, or ``fixed_step_size`` as a constant step; the TV /
-    low-pass object filters are not built."""
+    reference builds them from its simulation config); ``data`` are intensities,
+    or exit waves with ``phase_retrieval=False``; the step is 0.5 / Lipschitz, or
+    ``fixed_step_size`` as a constant step; the TV / low-pass object filters are
+    not built."""
 
     def __init__(self, simulation_space, data, probes, phase_retrieval: bool = True, solver_type: str = "iterative",
                  bc_type="impedance", verbose: Optional[bool] = None, device=None):
         if solver_type not in {"full", "iterative"}:
             raise ValueError(f"Invalid solver_type: {solver_type!r}")
-        if not phase_retrieval:
-            raise NotImplementedError("exit-wave (non phase-retrieval) data are not built for this driver")
         ss = simulation_space
         if ss.dimension != 2:
             raise NotImplementedError("two transverse axes: thick_ptycho_amd.recon3d.Reconstruction3D")
         self.ss = ss
         self.rc = Reconstruction2D(np.asarray(data).reshape(-1, ss.nx), np.asarray(probes).reshape(-1, ss.nx), ss.k,
-                                   ss.dx, ss.dz, bc_type, device=device)
+                                   ss.dx, ss.dz, bc_type, device=device, phase_retrieval=phase_retrieval)
 
     def reconstruct(self, max_iters: int = 10, tol: float = 1e-8, n_initial=None, fixed_step_size=None,
                     tv_lambda_amp: float = 0.0, low_pass_sigma_phase: float = 0.0):
