@@ -61,8 +61,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: every box in this environment has a single MI355X and the probe-sharded "
-            "multi-GPU path is not built (see DESIGN.md for how it would shard).",
+            "bench.py measures one GPU: the probe-sharded multi-GPU path (probes split across GPUs, one "
+            "all-reduce of the gradient per iteration) is not built.",
             file=sys.stderr,
         )
         return 2

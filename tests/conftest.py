@@ -24,7 +24,7 @@ def gpu_available() -> bool:
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free GPU gate: gpu tests must run on the GPU box; here they are deselected by -m."""
+    """GPU gate: tests marked gpu need a ROCm GPU and the built library; without a GPU they skip."""
     if not gpu_available():
         pytest.skip("no GPU in this environment")
     from thick_ptycho_amd import _lib
