@@ -11,7 +11,6 @@
 """
 
 import numpy as np
-import pytest
 
 from oracle import multislice as oms
 from oracle import pie

@@ -15,8 +15,6 @@ stepping kernels (tp_tri_sweep / tp_band_sweep) with the per-slice factors of
 
 from __future__ import annotations
 
-from typing import Optional
-
 import numpy as np
 import torch
 

@@ -19,7 +19,7 @@ import torch
 
 from . import band_engine, tri_engine
 from .base_solver import BasePWESolver
-from .operators import BoundaryType, Tridiag, chunk_for, step_operators_1d
+from .operators import BoundaryType, chunk_for, step_operators_1d
 
 
 class PWEForwardModel:

@@ -9,7 +9,6 @@ quirk Q6).  Only one transverse axis, like the reference (3D raises).
 from __future__ import annotations
 
 import time
-from typing import Optional
 
 import numpy as np
 import torch

@@ -9,9 +9,8 @@ one launch) and ``step`` the plain one-launch-per-slice form.
 
 from __future__ import annotations
 
-import ctypes
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 import torch

@@ -13,7 +13,6 @@ dx = dy = 2, dz = 4, n_medium = 1.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional
 
 import numpy as np
 
