@@ -57,3 +57,16 @@ def test_full_lu_two_axes(gpu, mode):
     for p in range(P):
         ref = pwe.full_lu_solve(g, "dirichlet", n, probes[p], mode)
         assert rel_l2(got[p], ref) < 1e-4, p
+
+
+@pytest.mark.parametrize("nx,nz", [(37, 2), (1000, 3)])
+def test_full_lu_edge_sizes(gpu, nx, nz, rng):
+    n = random_object(nx, nz, rng, amp=1e-2)
+    probes = rng.standard_normal((2, nx)) + 1j * rng.standard_normal((2, nx))
+    ss = SimulationSpace(nx, nz, 2.0, 4.0, wavelength=2 * np.pi / K, num_probes=2)
+    s = PWEFullLUSolver(ss, probes[None], bc_type="neumann")
+    g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=K)
+    for mode in ("forward", "adjoint"):
+        got = s.solve(n=n, mode=mode)[0, 0]
+        for p in range(2):
+            assert rel_l2(got[p], pwe.full_lu_solve(g, "neumann", n, probes[p], mode)) < 1e-4, (mode, p)

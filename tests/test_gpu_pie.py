@@ -57,3 +57,14 @@ def test_3pie_exit_waves_and_reduced_windows(gpu):
     print(f"reduced windows, exit-wave data: loss {h_gpu} vs {h_ref}; object update rel-L2 {d:.2e}")
     assert np.allclose(h_gpu, h_ref, rtol=1e-4)
     assert d < 1e-4
+
+
+def test_3pie_single_scan_two_slices(gpu):
+    nx, nz = 64, 2
+    n_true, probes, ex = _case(nx, nz, S=1, seed=9)
+    I = np.abs(np.fft.fft(ex, axis=-1)) ** 2
+    ss = SimulationSpace(nx, nz, DX, DZ, wavelength=2 * np.pi / K, num_probes=1)
+    n_gpu, h_gpu = reconstruct.ReconstructorMS(ss, I, probes).reconstruct(max_iters=2, alpha_obj=0.1)
+    n_ref, h_ref = pie.reconstruct_3pie(probes, I, nx, nz, DX, K, DZ, max_iters=2, alpha_obj=0.1)
+    assert np.allclose(h_gpu, h_ref, rtol=1e-4, atol=1e-7)
+    assert rel_l2(n_gpu - 1.0, n_ref - 1.0) < 1e-4

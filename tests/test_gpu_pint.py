@@ -196,3 +196,17 @@ def test_petsc_form_reaches_absolute_tolerance(gpu):
     g = _grid(nx, nz)
     for p in range(P):
         assert rel_l2(got[p], pwe.full_lu_solve(g, "impedance", n, probes[p])) < 1e-5
+
+
+@pytest.mark.parametrize("nx,nz", [(37, 2), (100, 3), (130, 9)])
+def test_edge_sizes(gpu, nx, nz):
+    """Widths that leave padded rows in the block solves, one and two slices."""
+    rng = np.random.default_rng(nx + nz)
+    n = random_object(nx, nz, rng, amp=1e-2)
+    probes = rng.standard_normal((2, nx)) + 1j * rng.standard_normal((2, nx))
+    s = _solver(nx, nz, probes, "impedance", rtol=1e-10, atol=0.0)
+    got = s.solve(n=n)[0, 0]
+    assert s.last_result.converged.all()
+    g = _grid(nx, nz)
+    for p in range(2):
+        assert rel_l2(got[p], pwe.full_lu_solve(g, "impedance", n, probes[p])) < 1e-7
