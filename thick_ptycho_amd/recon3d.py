@@ -34,7 +34,10 @@ from .forward_model import band_engine as be
 class Reconstruction3D:
     def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
                  batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None,
-                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6):
+                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6, exit_data=None):
+        """``sqrt_i`` (P, nx, ny): measured far-field amplitudes; or ``sqrt_i=None`` with
+        ``exit_data`` (P, nx, ny) complex exit waves and the loss sum |u_L - exit_data|^2
+        (the reference's non phase-retrieval data; iterations then run eagerly)."""
         dev = device or torch.device("cuda", torch.cuda.current_device())
         self.dev = dev
         n0 = np.asarray(n0)
@@ -48,7 +51,12 @@ class Reconstruction3D:
         self.batch = batch or P
         self.n = torch.tensor(n0, dtype=torch.complex128, device=dev)
         self.probes = torch.as_tensor(probes).to(dev, torch.complex64).reshape(P, -1).contiguous()
-        self.sqrt_i = torch.as_tensor(sqrt_i).to(dev, torch.float32).reshape(P, nx, ny).contiguous()
+        if (sqrt_i is None) == (exit_data is None):
+            raise ValueError("give either sqrt_i (intensity data) or exit_data (exit waves)")
+        self.sqrt_i = None if sqrt_i is None else \
+            torch.as_tensor(sqrt_i).to(dev, torch.float32).reshape(P, nx, ny).contiguous()
+        self.exit_data = None if exit_data is None else \
+            torch.as_tensor(exit_data).to(dev, torch.complex64).reshape(P, nx * ny).contiguous()
         A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
         self.C = torch.empty((self.N, self.L), dtype=torch.complex128, device=dev)
         self._contribution()
@@ -109,9 +117,13 @@ class Reconstruction3D:
                 be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb, schedule=self.schedule)
             ex = self.exit[:nb]
             ex.copy_(u[:, self.L])
-            _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb, 1.0,
-                                               _lib.ptr(self.seed[:nb]), _lib.ptr(self.loss[b0:b1]),
-                                               _lib.ptr(self.work), sh()), "tp_amplitude_seed2d")
+            if self.exit_data is not None:  # J = sum |u_L - d|^2: seed u_L - d
+                torch.sub(ex, self.exit_data[b0:b1], out=self.seed[:nb, 0])
+                self.loss[b0:b1] = (self.seed[:nb, 0].abs() ** 2).sum(dim=1)
+            else:
+                _lib.check(lib.tp_amplitude_seed2d(_lib.ptr(ex), _lib.ptr(self.sqrt_i[b0:b1]), self.nx, self.ny, nb,
+                                                   1.0, _lib.ptr(self.seed[:nb]), _lib.ptr(self.loss[b0:b1]),
+                                                   _lib.ptr(self.work), sh()), "tp_amplitude_seed2d")
             rhs = self.seed[:nb] if seed is None else seed[b0:b1].reshape(nb, 1, self.N)
             be.sweep([be.BandScanSpec(adj, self.zero[:nb], out=lam, rhs=rhs, rhs_steps=1)], self.ppb,
                      schedule=self.schedule)
@@ -180,7 +192,7 @@ class Reconstruction3D:
         if alpha is None:
             alpha = 0.5 / self.lipschitz()
         self.alpha.fill_(alpha)
-        graph = graph and self.krylov is None
+        graph = graph and self.krylov is None and self.exit_data is None  # torch reductions stay out of graphs
         if graph and self.graph is None:
             self.capture()
         hist: List[float] = []
