@@ -57,6 +57,9 @@
  *                      (learned-preconditioned) alternative to the per-slice
  *                      LU solve of iterative_lu.py:185-193 in 3D, with the
  *                      banded factor above as its fallback.
+ *   tp_window_grad_accumulate
+ *                   <- ReconstructorPWE.accumulate_gradient (pwe_reconstructor.py:144-164),
+ *                      the window sum of reduced-domain gradients.
  *   tp_pie_adjoint_transmission, tp_pie_update (with tp_ms_sweep, conj_h)
  *                   <- ReconstructorMS._apply_backprop_updates / _update_object_n
  *                      (reconstruction/ms_reconstructor.py:109-190), 3PIE.
@@ -433,6 +436,16 @@ int tp_pint_matvec(const void* d_ops, const void* d_C, int64_t c_pstride, const 
  *   sum_e |w[p, e]|^2 (real part) in slot nrows; the caller sums over b.
  * tp_pint_combine writes out[p, e] = w[p, e] - sum_{i < nrows} V[p, i, e] c[p, i]
  *   (w null: 0; w may alias out). */
+/* Reduced-domain gradient accumulation, the per-scan window sum of
+ * ReconstructorPWE.accumulate_gradient (reconstruction/pwe_reconstructor.py:144-164)
+ * for the exact gradient.  u, lam: (S, L + 1, W) complex64, the forward planes
+ * of every scan and its exact-adjoint planes in stepping order (plane k holds
+ * lambda_{L-k+1}); win: S int32 first rows of the windows (a_s + W <= nx);
+ * G: (L, nx) complex64, overwritten with
+ *   G[j-1][x] = sum_{s: a_s <= x < a_s + W} conj(lam^s_j) (u^s_j + u^s_{j-1}) at row x - a_s. */
+int tp_window_grad_accumulate(const void* d_u, const void* d_lam, const int* d_win, int S, int L, int W, int nx,
+                              void* d_G, void* stream);
+
 /* ---------------------------------------------------------------- multislice 3PIE
  * ReconstructorMS._apply_backprop_updates / _update_object_n
  * (reconstruction/ms_reconstructor.py:109-190) for one scan position, split in

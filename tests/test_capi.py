@@ -79,6 +79,8 @@ def test_new_entry_points_reject_bad_arguments():
     assert lib.tp_pint_dots(1, 0, 0, 33, 1, 0, 10, 1, 1, 1, None) == _lib.TP_ERR_INVALID
     assert lib.tp_pie_update(1, 4, 8, 8, 1, 1, 1.0, 1.0, 1.0, 0.1, 1e-12, None) == _lib.TP_ERR_INVALID  # ld < nz
     assert lib.tp_pie_adjoint_transmission(None, 8, 8, 8, 1.0, 1.0, 1.0, 1, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_window_grad_accumulate(1, 1, None, 2, 4, 8, 16, 1, None) == _lib.TP_ERR_INVALID
+    assert lib.tp_window_grad_accumulate(1, 1, 1, 2, 4, 32, 16, 1, None) == _lib.TP_ERR_INVALID  # W > nx
     assert lib.tp_pint_combine(None, 0, 0, 2, None, 0, None, 0, 1, 0, 10, 1, None) == _lib.TP_ERR_INVALID
     # Krylov / CNN
     assert lib.tp_bicgstab_solve(1, 1, 1, 8, 8, 2, 1, 1, 3, None, 1e-6, 10, 0, 1, None, None) == _lib.TP_ERR_INVALID

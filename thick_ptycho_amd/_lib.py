@@ -196,6 +196,8 @@ SIGNATURES = {
     "tp_grad3d_accumulate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int64, c_void_p, c_void_p]),
     "tp_pint_matvec": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                                c_void_p]),
+    "tp_window_grad_accumulate": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                          c_void_p]),
     "tp_pie_adjoint_transmission": (c_int, [c_void_p, c_int64, c_int, c_int, c_double, c_double, c_double, c_void_p,
                                             c_void_p]),
     "tp_pie_update": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,

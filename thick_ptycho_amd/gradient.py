@@ -145,6 +145,67 @@ def amplitude_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="di
     return float(loss.double().sum().item()), 2 * torch.conj(Gn)
 
 
+def amplitude_gradient_reduced(n, probes, sqrt_i, windows, k: float, dx: float, dz: float, bc="dirichlet",
+                               device=None, exit_data=None):
+    """(loss, grad) for reduced-domain scans: scan s sees only rows [a_s, b_s) of
+    n (nx, nz), with probes / sqrt_i (or ``exit_data``) of the window width W;
+    J = sum_s J_s and grad complex128 (nx, nz) = the window sum of the per-scan
+    exact gradients (the accumulation of pwe_reconstructor.py:144-164 over
+    simulation_space_2d.py:80-135 windows).
+
+    Per-scan forward and exact-adjoint factor sets go into two buffers (eight
+    factorisations per launch), one persistent sweep steps every scan through
+    its own set in each direction keeping all planes, and
+    tp_window_grad_accumulate sums the scans' slice products into the full
+    width."""
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    n_t = _dev_tensor(n, torch.complex128, device)
+    nx, nz = n_t.shape
+    L = nz - 1
+    win = [(int(a), int(b)) for a, b in windows]
+    S = len(win)
+    W = win[0][1] - win[0][0]
+    if any(b - a != W or a < 0 or b > nx for a, b in win):
+        raise ValueError("windows must all have the same width and lie inside the object")
+    A, B = _operators(W, k, dx, dz, bc)
+    C = object_contribution(n_t, k, dz)  # (nx, L); a window's rows are its object term
+    chunk = operators.chunk_for(W)
+    per = tri_engine.coef_elems(L, chunk)
+    whole = {}
+    for mode in ("forward", "exact_adjoint"):
+        coef = torch.empty((S, per), dtype=torch.complex64, device=device)
+        first = None
+        for i in range(0, S, _lib.TP_MAX_SCANS):
+            jobs = [(C[a:b], A, B, mode) for a, b in win[i:i + _lib.TP_MAX_SCANS]]
+            facs = tri_engine.factor_many(jobs, chunk, out=[coef[s] for s in range(i, min(i + _lib.TP_MAX_SCANS, S))])
+            first = first or facs[0]
+        whole[mode] = tri_engine.TriFactors(coef.reshape(-1), W, L, chunk, mode, first.b_sign, first.status,
+                                            first.sigma, keep=[coef, facs])
+    u0 = _dev_tensor(probes, torch.complex64, device).reshape(S, W)
+    U = torch.empty((S, L + 1, W), dtype=torch.complex64, device=device)
+    tri_engine.sweep([tri_engine.TriScanSpec(whole["forward"], u0, out=U, coef_pstride=per)], probes_per_block=1)
+    ex = U[:, L].contiguous()
+    if exit_data is not None:
+        sd = ex - _dev_tensor(exit_data, torch.complex64, device).reshape(S, W)
+        loss = (sd.abs() ** 2).sum(dim=1)
+    else:
+        sd, loss = amplitude_seed(ex, _dev_tensor(sqrt_i, torch.float32, device).reshape(S, W), 1.0)
+    Lam = torch.empty_like(U)
+    zero = torch.zeros((S, W), dtype=torch.complex64, device=device)
+    tri_engine.sweep([tri_engine.TriScanSpec(whole["exact_adjoint"], zero, out=Lam, rhs=sd.reshape(S, 1, W).contiguous(),
+                                             rhs_steps=1, coef_pstride=per)], probes_per_block=1)
+    wins = torch.tensor([a for a, _ in win], dtype=torch.int32, device=device)
+    G = torch.empty((L, nx), dtype=torch.complex64, device=device)
+    _lib.check(_lib.load().tp_window_grad_accumulate(_lib.ptr(U), _lib.ptr(Lam), _lib.ptr(wins), S, L, W, nx,
+                                                       _lib.ptr(G), _lib.stream_handle()), "tp_window_grad_accumulate")
+    Gn = torch.zeros((nx, nz), dtype=torch.complex128, device=device)
+    Gt = G.to(torch.complex128).transpose(0, 1)
+    Gn[:, :-1] += Gt
+    Gn[:, 1:] += Gt
+    Gn *= (dz / 4) * (k / 1j)
+    return float(loss.double().sum().item()), 2 * torch.conj(Gn)
+
+
 def reference_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="dirichlet", checkpoint: int = 32,
                        device=None, exit_error=None):
     """(grad_real, grad_imag), each float64 (L*N,) step-major, as
