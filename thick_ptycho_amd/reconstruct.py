@@ -65,10 +65,13 @@ def beta_rule(g_new: torch.Tensor, g_old: torch.Tensor) -> float:
 class Reconstruction2D:
     """Reconstruct n (nx, nz) from far-field intensities I (P, nx) of probes (P, nx),
     or, with ``phase_retrieval=False``, from complex exit waves (P, nx) with the
-    loss sum |u_L - data|^2 (the reference's two data kinds, pwe_reconstructor.py)."""
+    loss sum |u_L - data|^2 (the reference's two data kinds, pwe_reconstructor.py).
+    With ``windows`` [(a_s, b_s)] (reduced domain) probe s and its data are the
+    window width and see rows a_s..b_s-1 of n only."""
 
     def __init__(self, intensity, probes, k: float, dx: float, dz: float, bc: str = "dirichlet",
-                 checkpoint: int = 32, device=None, phase_retrieval: bool = True):
+                 checkpoint: int = 32, device=None, phase_retrieval: bool = True, windows=None):
+        self.windows = None if windows is None else [(int(a), int(b)) for a, b in windows]
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.probes = torch.as_tensor(np.asarray(probes)).to(self.dev, torch.complex64)
         self.phase_retrieval = phase_retrieval
@@ -81,18 +84,31 @@ class Reconstruction2D:
         self.k, self.dx, self.dz, self.bc, self.checkpoint = k, dx, dz, bc, checkpoint
 
     def gradient(self, n: torch.Tensor):
+        if self.windows is not None:
+            return tg.amplitude_gradient_reduced(n, self.probes, self.sqrt_i, self.windows, self.k, self.dx, self.dz,
+                                                 self.bc, device=self.dev, exit_data=self.exit_data)
         return tg.amplitude_gradient(n, self.probes, self.sqrt_i, self.k, self.dx, self.dz, self.bc,
                                      self.checkpoint, device=self.dev, exit_data=self.exit_data)
 
     def wave_field(self, n) -> np.ndarray:
         """Forward field of every probe at slices 1..nz-1, complex64 (1, P, nx, nz-1): the
-        reference's ``convert_to_tensor_form(uk)`` layout (base_reconstructor.py:123-140)."""
+        reference's ``convert_to_tensor_form(uk)`` layout (base_reconstructor.py:123-140);
+        with windows each probe's field on its window, (1, P, W, nz-1)."""
         n_t = tg._dev_tensor(n, torch.complex128, self.dev)
+        if self.windows is not None:
+            return np.stack([self._window_field(n_t[a:b], p) for p, (a, b) in enumerate(self.windows)])[None]
         N, nz = n_t.shape
         A, B = tg._operators(N, self.k, self.dx, self.dz, self.bc)
         sw = tg._Sweeper(tg.object_contribution(n_t, self.k, self.dz), A, B, "exact_adjoint", 1)
         ck, _ = sw.forward(self.probes.reshape(-1, N))
         return ck[:, 1:].transpose(1, 2).unsqueeze(0).cpu().numpy()
+
+    def _window_field(self, n_w: torch.Tensor, p: int) -> np.ndarray:
+        N, nz = n_w.shape
+        A, B = tg._operators(N, self.k, self.dx, self.dz, self.bc)
+        sw = tg._Sweeper(tg.object_contribution(n_w.contiguous(), self.k, self.dz), A, B, "exact_adjoint", 1)
+        ck, _ = sw.forward(self.probes[p:p + 1].reshape(1, N).contiguous())
+        return ck[0, 1:].transpose(0, 1).cpu().numpy()
 
     def lipschitz(self, n: torch.Tensor, h: float = 1e-5) -> float:
         _, g0 = self.gradient(n)
@@ -161,8 +177,11 @@ class ReconstructorPWE:
         if ss.dimension != 2:
             raise NotImplementedError("two transverse axes: thick_ptycho_amd.recon3d.Reconstruction3D")
         self.ss = ss
-        self.rc = Reconstruction2D(np.asarray(data).reshape(-1, ss.nx), np.asarray(probes).reshape(-1, ss.nx), ss.k,
-                                   ss.dx, ss.dz, bc_type, device=device, phase_retrieval=phase_retrieval)
+        windows = list(getattr(ss, "_scan_windows", None) or []) if ss.solve_reduced_domain else None
+        W = ss.effective_nx if windows else ss.nx
+        self.rc = Reconstruction2D(np.asarray(data).reshape(-1, W), np.asarray(probes).reshape(-1, W), ss.k,
+                                   ss.dx, ss.dz, bc_type, device=device, phase_retrieval=phase_retrieval,
+                                   windows=windows)
 
     def reconstruct(self, max_iters: int = 10, tol: float = 1e-8, n_initial=None, fixed_step_size=None,
                     tv_lambda_amp: float = 0.0, low_pass_sigma_phase: float = 0.0):
