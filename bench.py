@@ -259,6 +259,17 @@ def main() -> int:
             "ms_per_pass": t_r * 1e3,
         }
         log(f"reduced domain: {t_r * 1e3:.2f} ms per pass, {extras['reduced_domain']['value']:.4g} steps/s")
+        from thick_ptycho_amd import gradient as tgr
+
+        W_r = ss_r.effective_nx
+        wins_r = list(ss_r._scan_windows)
+        # exit-wave data (the window width, 255, is not a far-field transform length); synthetic: the probes
+        t_g = timed(lambda: tgr.amplitude_gradient_reduced(n_r, pr_r, None, wins_r, ss_r.k, 2.0, 4.0, "impedance",
+                                                           exit_data=pr_r), 2, 1)
+        extras["reduced_domain"]["gradient_ms"] = t_g * 1e3
+        extras["reduced_domain"]["gradient_value"] = 2 * S_r * (nz_r - 1) / t_g
+        log(f"reduced-domain gradient (window width {W_r}): {t_g * 1e3:.2f} ms, "
+            f"{extras['reduced_domain']['gradient_value']:.4g} steps/s (fwd+adj incl. factorisation, exit-wave loss)")
         del sol_r
 
         # parallel in time: whole-system GMRES with the alpha-circulant preconditioner
