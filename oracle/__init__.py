@@ -29,5 +29,15 @@ Pinning (how each function is checked independently of its own formula):
     with the exact one and it must equal a closed form in the finite-difference
     pinned exact adjoint states (tests/test_oracle_gradient.py); the z-varying
     case rests on the restatement.  The exact gradient is pinned by finite
-    differences.
+    differences (amplitude and exit-wave losses); the reduced-domain gradient
+    is the sum of per-window exact gradients, each finite-difference pinned.
+  * parallel-in-time GMRES (``pint``): the preconditioner against a sparse
+    solve of the assembled alpha-circulant matrix, the solution against the
+    whole-system SuperLU solve in both modes.
+  * 3PIE (``pie``): modulus-projection invariants, back-propagation as the
+    conjugate transpose of the dense propagation matrix, falling loss.
+  * learned preconditioner and BiCGSTAB (``learned``): ``correlate2d``,
+    complex arithmetic, finite differences, ``spsolve`` and scipy's bicgstab.
+  * streaming 3D reconstruction (``recon3d``): SuperLU on the assembled slice
+    operators and the finite-difference pinned ``exact_gradient_3d``.
 """
