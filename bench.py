@@ -234,6 +234,30 @@ def main() -> int:
         extras["multislice_gradient_ms"] = t_g * 1e3
         log(f"multislice gradient: {t_g * 1e3:.2f} ms, {extras['multislice_gradient_value']:.4g} steps/s (fwd+adj)")
 
+    if not args.no_extras:  # the drop-in solve(): full complex128 fields handed over in host memory
+        from thick_ptycho_amd import SimulationSpace
+        from thick_ptycho_amd.forward_model import PWEIterativeLUSolver
+
+        nx_h, nz_h, P_h = 1024, 512, 256
+        ss_h = SimulationSpace(nx_h, nz_h, 2.0, 4.0, num_probes=P_h)
+        n_h = workloads.blob_object_2d(nx_h, nz_h, 24, np.random.default_rng(70))
+        pr_h = workloads.gaussian_probes_1d(nx_h, np.linspace(nx_h / 4, 3 * nx_h / 4, P_h), 24.0)
+        sol_h = PWEIterativeLUSolver(ss_h, pr_h[None], bc_type="impedance")
+        t_dev = timed(lambda: sol_h.solve_device(n=n_h), 3, 1)
+        t_host = timed(lambda: sol_h.solve(n=n_h), 2, 1)
+        f_bytes = P_h * nx_h * nz_h * 16
+        extras["host_api"] = {
+            "workload": f"drop-in solve(), {nx_h}x{nz_h} x {P_h} probes, forward incl. factorisation, "
+                        "complex128 (S, N, nz) fields returned in host memory",
+            "device_ms": t_dev * 1e3,
+            "host_ms": t_host * 1e3,
+            "value_host": P_h * (nz_h - 1) / t_host,
+            "handover_gb_per_s": f_bytes / (t_host - t_dev) / 1e9,
+        }
+        log(f"drop-in solve() with host hand-over: {t_host * 1e3:.1f} ms ({t_dev * 1e3:.2f} ms on the device, "
+            f"{f_bytes / 1e9:.2f} GB of complex128 fields at {extras['host_api']['handover_gb_per_s']:.1f} GB/s)")
+        del sol_h
+
     if not args.no_extras:  # reduced domain: every scan position on its own window of the object
         from thick_ptycho_amd import SimulationSpace
         from thick_ptycho_amd.forward_model import PWEIterativeLUSolver

@@ -25,6 +25,34 @@ import torch
 from .. import _lib
 from . import ms_engine
 
+HOST_CHUNK_BYTES = 1 << 28  # complex128 bytes converted on the device per host copy
+
+
+def host_complex128(u: torch.Tensor) -> np.ndarray:
+    """complex128 host array with ``u``'s shape and C order, ``u`` any (strided) view.
+
+    The reference returns complex128 fields on the host (base/base_solver.py:180-188).
+    The transpose and the complex64 -> complex128 widening run on the device,
+    a chunk of the leading axes at a time, and each chunk is copied into
+    page-locked host memory (the device-to-host copy then runs at the link's DMA
+    rate instead of through pageable staging and a single-threaded host cast);
+    where page-locked memory cannot be had the chunks go into ordinary memory.
+    """
+    shape = tuple(u.shape)
+    try:
+        host = torch.empty(shape, dtype=torch.complex128, pin_memory=u.is_cuda)
+    except RuntimeError:
+        host = torch.empty(shape, dtype=torch.complex128)
+    if u.numel() == 0:
+        return host.numpy()
+    inner = shape[-2:] if len(shape) >= 2 else shape
+    lead = u.reshape(-1, *inner)
+    dst = host.view(-1, *inner)
+    per = max(1, HOST_CHUNK_BYTES // (16 * max(1, int(np.prod(inner)))))
+    for i in range(0, lead.shape[0], per):
+        dst[i:i + per].copy_(lead[i:i + per].to(torch.complex128))
+    return host.numpy()
+
 
 def _setup_log(verbose: bool):
     logger = logging.getLogger("thick_ptycho_amd")

@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from . import band_engine, tri_engine
+from .base_solver import host_complex128
 from .iterative_lu import PWEIterativeLUSolver
 from .operators import BoundaryType
 
@@ -117,7 +118,7 @@ class PWEFullLUSolver(PWEIterativeLUSolver):
     def solve(self, n=None, mode: str = "forward", rhs_block=None, probes=None, **kwargs) -> np.ndarray:
         """Reference-compatible solve: complex128 (Proj, A, S, *effective_shape) on the host."""
         u = self.solve_device(n=n, mode=mode, rhs_block=rhs_block, probes=probes)
-        u = u.transpose(-1, -2).cpu().numpy().astype(np.complex128)
+        u = host_complex128(u.transpose(-1, -2))
         return u.reshape(self.num_projections, self.num_angles, self.num_probes, *self.effective_shape)
 
     def _solve_single_probe_impl(self, scan_idx: int = 0, proj_idx: int = 0, probe=None, mode: str = "forward",

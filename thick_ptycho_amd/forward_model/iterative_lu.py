@@ -18,7 +18,7 @@ import scipy.sparse as sp
 import torch
 
 from . import band_engine, tri_engine
-from .base_solver import BasePWESolver
+from .base_solver import BasePWESolver, host_complex128
 from .operators import BoundaryType, chunk_for, step_operators_1d
 
 
@@ -292,5 +292,5 @@ class PWEIterativeLUSolver(BasePWESolver):
         (Proj, A, S, *effective_shape) on the host."""
         assert mode in {"forward", "adjoint", "reverse"}, f"Invalid mode: {mode!r}"
         u = self.solve_device(n=n, mode=mode, rhs_block=rhs_block, probes=probes, keep="all")
-        u = u.transpose(-1, -2).cpu().numpy().astype(np.complex128)
+        u = host_complex128(u.transpose(-1, -2))
         return u.reshape(self.num_projections, self.num_angles, self.num_probes, *self.effective_shape)

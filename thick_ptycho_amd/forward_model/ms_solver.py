@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import ms_engine
-from .base_solver import BaseForwardModelSolver
+from .base_solver import BaseForwardModelSolver, host_complex128
 
 
 class MSForwardModelSolver(BaseForwardModelSolver):
@@ -87,7 +87,7 @@ class MSForwardModelSolver(BaseForwardModelSolver):
         (base/base_solver.py:113 and ms_solver.py:122-125)."""
         assert mode in {"forward", "adjoint", "reverse"}, f"Invalid mode: {mode!r}"
         u = self.solve_device(n=n, mode=mode, probes=probes, keep="all")
-        return u.transpose(-1, -2).cpu().numpy().astype(np.complex128)
+        return host_complex128(u.transpose(-1, -2))
 
     def _solve_single_probe(self, scan_idx: int = 0, proj_idx: int = 0, n=None, mode="forward", probe=None,
                             **kwargs) -> np.ndarray:

@@ -48,7 +48,7 @@ import torch
 
 from .. import _lib
 from . import tri_engine
-from .base_solver import BasePWESolver
+from .base_solver import BasePWESolver, host_complex128
 from .iterative_lu import PWEForwardModel
 from .operators import BoundaryType, Tridiag, chunk_for
 
@@ -396,7 +396,7 @@ class PWEFullPinTSolver(BasePWESolver):
         """Reference-compatible solve (base/base_solver.py:84-154): complex128
         (Proj, A, S, nx, nz) on the host."""
         u = self.solve_device(n=n, mode=mode, rhs_block=rhs_block, probes=probes)
-        u = u.transpose(-1, -2).cpu().numpy()
+        u = host_complex128(u.transpose(-1, -2))
         return u.reshape(self.num_projections, self.num_angles, self.num_probes, *self.effective_shape)
 
 
