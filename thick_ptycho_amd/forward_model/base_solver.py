@@ -37,6 +37,8 @@ def host_complex128(u: torch.Tensor) -> np.ndarray:
     page-locked host memory (the device-to-host copy then runs at the link's DMA
     rate instead of through pageable staging and a single-threaded host cast);
     where page-locked memory cannot be had the chunks go into ordinary memory.
+    torch's host allocator caches page-locked blocks (sizes rounded up to a power
+    of two) for reuse once the returned array is freed.
     """
     shape = tuple(u.shape)
     try:
