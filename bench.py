@@ -402,7 +402,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"tri_sweep_kernel<16,{args.ppb},false> (forward+adjoint, 8 batches)",
+            "kernel": f"tri_sweep_kernel<16,{args.ppb},false,false,false> (forward+adjoint, 8 batches)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -277,9 +277,11 @@ struct NoHook {
   __device__ __forceinline__ void operator()(int) const {}
 };
 
-template <int CH, bool HAS_RHS, bool SIGMA, typename Ptr, typename Hook = NoHook>
-__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
-                                              const Hook& hook = Hook()) {
+// Direct form: every coefficient is read where it is used (the register
+// budget of CH = 32 holds no second bank).
+template <int CH, bool HAS_RHS, bool SIGMA, typename Ptr, typename Hook>
+__device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
+                                                     const Hook& hook) {
   constexpr int NP = 64 * CH;
   float2 y[CH];
 #pragma unroll
@@ -314,6 +316,78 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
 #pragma unroll
     for (int m = 0; m < CH; ++m) {
       const float2 x = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
+      u[m] = SIGMA ? csub(x, u[m]) : make_float2(fmaf(2.f, x.x, -u[m].x), fmaf(2.f, x.y, -u[m].y));
+    }
+  }
+  hook(3);
+}
+
+// Staged form (CH <= 16): the coefficients of each phase are read into
+// registers one phase ahead of their use (two banks of CH values), and
+// scheduling barriers keep the compiler from sinking the reads next to their
+// uses, so a phase waits on reads issued a phase earlier instead of on one read
+// every few instructions.  This pays where a SIMD holds one wavefront and
+// nothing else covers the waits; with two wavefronts per SIMD the direct form's
+// freer schedule is faster.
+template <int CH, bool HAS_RHS, bool SIGMA, bool STAGED = true, typename Ptr, typename Hook = NoHook>
+__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
+                                              const Hook& hook = Hook()) {
+  if constexpr (CH > 16 || !STAGED) {
+    tri_step_lane_direct<CH, HAS_RHS, SIGMA>(u, cf, rhs, lane, hook);
+    return;
+  }
+  constexpr int NP = 64 * CH;
+  float2 y[CH], c1[CH], c2[CH];
+#pragma unroll
+  for (int m = 0; m < CH; ++m) {
+    c1[m] = cf[A_L * NP + m * 64 + lane];
+    c2[m] = cf[A_GF * NP + m * 64 + lane];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) {
+    float2 r = u[m];
+    if (SIGMA) r = cmul(cf[A_SG * NP + m * 64 + lane], r);
+    if (HAS_RHS) r = SIGMA ? cadd(r, rhs[m]) : make_float2(fmaf(0.5f, rhs[m].x, r.x), fmaf(0.5f, rhs[m].y, r.y));
+    y[m] = r;
+  }
+  hook(0);
+#pragma unroll
+  for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
+  hook(1);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
+  __builtin_amdgcn_sched_barrier(0);
+  {
+    float2 A = c2[CH - 1];
+    float2 B = y[CH - 1];
+    scan_affine_up(A, B);
+    const float2 yin = from_prev_lane(B);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) y[m] = cfma(c2[m], yin, y[m]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], c1[m]);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
+  hook(2);
+  {
+    float2 A = c1[0];
+    float2 B = y[0];
+    scan_affine_down(A, B, lane);
+    const float2 xin = from_next_lane(B);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+      const float2 x = cfma(c1[m], xin, y[m]);
       u[m] = SIGMA ? csub(x, u[m]) : make_float2(fmaf(2.f, x.x, -u[m].x), fmaf(2.f, x.y, -u[m].y));
     }
   }
@@ -372,8 +446,7 @@ __device__ __forceinline__ void store_plane(const tp_tri_scan& d, int p, int s, 
 // Coalesced plane stores: the wavefront's plane goes through an LDS tile
 // (row i at i + 2*(i/16): two pad words per 16 rows against bank conflicts,
 // row pairs stay 16-byte aligned) and leaves as 16-byte stores, each
-// instruction writing 64 consecutive row pairs (1 KiB).  The stores are split
-// into four groups that the step interleaves with its compute phases.
+// instruction writing 64 consecutive row pairs (1 KiB).
 __device__ __forceinline__ int tile_index(int i) { return i + 2 * (i >> 4); }
 
 template <int CH>
@@ -402,7 +475,8 @@ struct PlaneWriter {
     dst[2] = (act && d.exitp && s == d.steps) ? (float2*)d.exitp + (size_t)p * d.exit_pstride : nullptr;
   }
 
-  // group g of 4: row pairs q = lane + 64 k for k in [g*CH/8, (g+1)*CH/8)
+  // group g of 4: row pairs j = 2 (lane + 64 k) for k in [g*CH/8, (g+1)*CH/8),
+  // each read from the tile just before its store
   __device__ __forceinline__ void operator()(int g) const {
     constexpr int KP = CH / 2 > 0 ? CH / 2 : 1;  // pair-instructions per plane
     const int k0 = (g * KP) / 4, k1 = ((g + 1) * KP) / 4;
@@ -429,6 +503,38 @@ struct PlaneWriter {
     (*this)(2);
     (*this)(3);
   }
+
+  // Batched form: every LDS read of the plane is issued before the first store,
+  // so the stores wait on the LDS once rather than once each (faster where one
+  // wavefront per SIMD leaves the waits exposed, slower in full launches).
+  __device__ __forceinline__ void all_batched() const {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int NP = 64 * CH;
+    constexpr int KP = CH / 2 > 0 ? CH / 2 : 1;  // pair-instructions per plane
+    if (dst[0] == nullptr && dst[1] == nullptr && dst[2] == nullptr) return;
+    f4 v[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int j = 2 * (lane + 64 * k);
+      v[k] = (CH >= 2 || j < NP) ? *reinterpret_cast<const f4*>(tile + tile_index(j)) : f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      float2* o = dst[t];
+      if (o == nullptr) continue;
+      const bool aligned = (reinterpret_cast<uintptr_t>(o) & 15) == 0;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int j = 2 * (lane + 64 * k);
+        if (aligned && j + 1 < n) {
+          *reinterpret_cast<f4*>(o + j) = v[k];
+        } else {
+          if (j < n) o[j] = make_float2(v[k].x, v[k].y);
+          if (j + 1 < n) o[j + 1] = make_float2(v[k].z, v[k].w);
+        }
+      }
+    }
+  }
 };
 
 // ------------------------------------------------------------------ persistent sweep
@@ -439,7 +545,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // wavefronts; the next step's block is fetched into registers while the
 // current step computes, and the plane computed in step s is written at the
 // start of step s+1 (before that fetch) so the stores drain behind the math.
-template <int CH, int PPB, bool SIGMA, bool GRAD>
+template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
 __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
@@ -515,7 +621,7 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
       tile_put<CH>(tile, u, row0);
       PlaneWriter<CH> w;
       w.init(d, p, s - 1, act, tile, lane);
-      w.all();
+      if constexpr (STAGED) w.all_batched(); else w.all();
     }
     if (s < L) {
       const f4v* src = coef4 + (size_t)s * STRIDE_V;
@@ -535,9 +641,9 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
       const float2* r = (const float2*)d.rhs + (size_t)(act ? p : 0) * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
       for (int m = 0; m < CH; ++m) rb[m] = (act && row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-      tri_step_lane<CH, true, SIGMA>(u, cf_l, rb, lane);
+      tri_step_lane<CH, true, SIGMA, STAGED>(u, cf_l, rb, lane);
     } else {
-      tri_step_lane<CH, false, SIGMA>(u, cf_l, rb, lane);
+      tri_step_lane<CH, false, SIGMA, STAGED>(u, cf_l, rb, lane);
     }
     int grow = 0;
     if (GRAD) {
@@ -579,7 +685,7 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   {
     PlaneWriter<CH> w;
     w.init(d, p, L, act, tile, lane);
-    w.all();
+    if constexpr (STAGED) w.all_batched(); else w.all();
   }
 }
 
@@ -619,7 +725,7 @@ __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s)
   if (s == 1) {
     PlaneWriter<CH> w0;
     w0.init(d, p, 0, true, tile, lane);
-    w0.all();
+    w0.all_batched();
     __builtin_amdgcn_wave_barrier();
   }
   float2 rb[CH];
@@ -635,7 +741,7 @@ __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s)
   tile_put<CH>(tile, u, row0);
   PlaneWriter<CH> w;
   w.init(d, p, s, true, tile, lane);
-  w.all();
+  w.all_batched();
 }
 
 template <int CH, bool SIGMA>
@@ -711,6 +817,32 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
   return TP_OK;
 }
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
+static int launch_sweep_form(const SweepArgs& a, size_t lds, hipStream_t st) {
+  if (lds > 65536) {
+    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA, GRAD, STAGED>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA, GRAD, STAGED>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+// Launches of at most one workgroup per compute unit leave one wavefront per
+// SIMD and take the staged step; fuller launches take the direct one.
 template <int CH, int PPB, bool SIGMA, bool GRAD>
 static int launch_sweep(const SweepArgs& a, hipStream_t st) {
   constexpr int NP = 64 * CH;
@@ -721,13 +853,10 @@ static int launch_sweep(const SweepArgs& a, hipStream_t st) {
               CH, PPB, lds);
     return TP_ERR_INVALID;
   }
-  if (lds > 65536) {
-    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA, GRAD>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if constexpr (CH <= 16) {
+    if (a.nblk <= device_cus()) return launch_sweep_form<CH, PPB, SIGMA, GRAD, true>(a, lds, st);
   }
-  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA, GRAD>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
-  TP_CHECK_HIP(hipGetLastError());
-  return TP_OK;
+  return launch_sweep_form<CH, PPB, SIGMA, GRAD, false>(a, lds, st);
 }
 
 template <int CH, bool SIGMA>
