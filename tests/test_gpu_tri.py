@@ -191,3 +191,34 @@ def test_config1_batched_parity(gpu):
     tri_engine.sweep(specs[4:])
     for got, ref in zip(outs, refs):
         assert rel_l2(got.cpu().numpy(), ref) < TOL
+
+
+def test_staged_and_direct_step_forms_agree(gpu):
+    """The persistent sweep takes a staged step form (coefficients read a phase
+    ahead) when a launch has at most one workgroup per compute unit and the
+    direct form otherwise; both evaluate the same operations in the same order.
+    Config 1's eight (scan, direction) batches in one launch (448 workgroups of 4
+    probes) take the direct form, the two directions launched apart (224 each)
+    the staged one: the exit planes are bitwise equal."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    wl = workloads.config1()
+    specs, outs = [], []
+    for mode in ("forward", "adjoint"):
+        for sc in wl.scans:
+            g = pwe.Grid(dimension=2, nx=sc.nx, nz=sc.nz, dx=sc.dx, dz=sc.dz, k=sc.k)
+            A, B = operators.step_operators_1d(sc.nx, sc.k, sc.dx, sc.dz, sc.bc)
+            C = torch.from_numpy(pwe.object_contribution(g, sc.n)).to("cuda")
+            fac = tri_engine.factor(C, A, B, mode, chunk=16)
+            u0 = torch.from_numpy(sc.probes).to("cuda", torch.complex64)
+            outs.append((torch.empty_like(u0), torch.empty_like(u0)))
+            specs.append((fac, u0))
+    blocks_all = sum((u0.shape[0] + 3) // 4 for _, u0 in specs)
+    blocks_half = sum((u0.shape[0] + 3) // 4 for _, u0 in specs[:4])
+    assert blocks_half <= cus < blocks_all, (blocks_half, cus, blocks_all)
+    tri_engine.sweep([tri_engine.TriScanSpec(f, u0, exit=o[0]) for (f, u0), o in zip(specs, outs)],
+                     probes_per_block=4)
+    for half in (slice(0, 4), slice(4, 8)):
+        tri_engine.sweep([tri_engine.TriScanSpec(f, u0, exit=o[1]) for (f, u0), o in zip(specs[half], outs[half])],
+                         probes_per_block=4)
+    for direct, staged in outs:
+        torch.testing.assert_close(staged, direct, rtol=0, atol=0)
