@@ -38,7 +38,9 @@ def test_library_exports_all_declared_symbols():
 
 def test_abi_version_and_error_path():
     lib = _lib.load()
-    assert lib.tp_abi_version() == 1
+    header = open(os.path.join(ROOT, "include", "thick_ptycho_amd.h")).read()
+    declared = int(re.search(r"#define TP_ABI_VERSION (\d+)", header).group(1))
+    assert lib.tp_abi_version() == declared == _lib.ABI_VERSION
     # an invalid call fails on the host before touching the GPU
     rc = lib.tp_tri_factor(None, 10, 5, None, -1.0, 1.0, 0, 0, 0, 1, 5, 3, 0, None, None, None)
     assert rc == _lib.TP_ERR_INVALID

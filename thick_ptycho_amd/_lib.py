@@ -215,6 +215,9 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+ABI_VERSION = 1  # TP_ABI_VERSION of include/thick_ptycho_amd.h
+
+
 def load() -> ctypes.CDLL:
     """Load the HIP library (raises if it has not been built)."""
     global _lib
@@ -229,6 +232,10 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.tp_abi_version() != ABI_VERSION:
+        raise LibraryMissing(
+            f"{LIB_PATH} has ABI version {lib.tp_abi_version()}, these bindings expect {ABI_VERSION}: rebuild it"
+        )
     _lib = lib
     return lib
 
