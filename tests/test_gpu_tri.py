@@ -222,3 +222,21 @@ def test_staged_and_direct_step_forms_agree(gpu):
                          probes_per_block=4)
     for direct, staged in outs:
         torch.testing.assert_close(staged, direct, rtol=0, atol=0)
+
+
+def test_host_handover_in_chunks(gpu, rng, monkeypatch):
+    """solve() widens and transposes on the device a few planes at a time into
+    page-locked host memory; with a chunk of two probes' fields the complex128
+    result equals the device planes exactly."""
+    from thick_ptycho_amd.forward_model import base_solver
+
+    nx, nz = 200, 30
+    n = random_object(nx, nz, rng, amp=1e-2)
+    probes = rng.standard_normal((5, nx)) + 1j * rng.standard_normal((5, nx))
+    s = _solver(nx, nz, 2.0, 4.0, 2 * np.pi, probes, "impedance")
+    dev = s.solve_device(n=n)
+    monkeypatch.setattr(base_solver, "HOST_CHUNK_BYTES", 16 * nx * nz * 2)
+    h = s.solve(n=n)
+    assert h.dtype == np.complex128 and h.shape == (1, 1, 5, nx, nz)
+    ref = dev.transpose(-1, -2).cpu().numpy().astype(np.complex128)
+    np.testing.assert_array_equal(h, ref)
