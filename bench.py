@@ -53,7 +53,7 @@ def main() -> int:
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D (config 3) banded-LU and learned lines")
     ap.add_argument("--no-recon", action="store_true", help="skip the 3D reconstruction loop (256x256x128)")
     ap.add_argument("--recon-iters", type=int, default=50)
-    ap.add_argument("--cpu-probes", type=int, default=96)
+    ap.add_argument("--cpu-probes", type=int, default=128)
     ap.add_argument("--cpu-probes-per-proc", type=int, default=32)
     ap.add_argument("--ppb", default="auto", help="probes per workgroup of the persistent sweep, or 'auto'")
     ap.add_argument("--copy-calibration", action="store_true",
