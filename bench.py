@@ -33,9 +33,10 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the persistent sweep on this workload (nz = 1024,
-# 4 probes per workgroup), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a
-# 1 GiB copy in the same process (profiles/tri_sweep_batched_cfg1_traffic.md).
-MEASURED_SWEEP_TRAFFIC = {(1024, 4): 15.23e9}
+# 4 probes per workgroup, the 384-wide scan at 8 rows per lane), from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy in the same process
+# (profiles/tri_sweep_batched_cfg1_traffic.md).
+MEASURED_SWEEP_TRAFFIC = {(1024, 4, True): 16.40e9}
 
 
 def log(msg: str) -> None:
@@ -56,6 +57,8 @@ def main() -> int:
     ap.add_argument("--cpu-probes", type=int, default=128)
     ap.add_argument("--cpu-probes-per-proc", type=int, default=32)
     ap.add_argument("--ppb", default="auto", help="probes per workgroup of the persistent sweep, or 'auto'")
+    ap.add_argument("--no-split-chunks", dest="split_chunks", action="store_false",
+                    help="step every scan with 16 rows per lane (the 384-wide scan too)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
@@ -111,10 +114,18 @@ def main() -> int:
     steps_per_dir = sum(s["sc"].steps for s in scans)
     steps_per_pass = 2 * steps_per_dir
 
-    def make_specs():
-        # the eight factorisations (scan x direction) in one launch, then the sweeps
+    def make_specs(split: bool = True):
+        # the eight factorisations (scan x direction), one launch per chunk width
+        # (rows per lane: 16 for the 640/896/1024-wide scans, 8 for the 384-wide
+        # one, or 16 for all), then the sweeps
         jobs = [(s["C"], s["A"], s["B"], mode) for mode in ("forward", "adjoint") for s in scans]
-        facs = tri_engine.factor_many(jobs, chunk)
+        split = split and args.split_chunks
+        widths = [operators.chunk_for(s["sc"].nx) if split else chunk for s in scans + scans]
+        facs = [None] * len(jobs)
+        for w in sorted(set(widths), reverse=True):
+            idx = [i for i, c in enumerate(widths) if c == w]
+            for i, fac in zip(idx, tri_engine.factor_many([jobs[i] for i in idx], w)):
+                facs[i] = fac
         specs = []
         for (_, _, _, mode), fac, s in zip(jobs, facs, scans + scans):
             key = "out_f" if mode == "forward" else "out_a"
@@ -123,8 +134,9 @@ def main() -> int:
 
     def one_pass(plain: bool = False):
         # forward and adjoint of every scan are independent: one persistent
-        # launch walks all eight (scan, direction) batches
-        specs = make_specs()
+        # launch walks all eight (scan, direction) batches; the plain form takes
+        # one chunk width per launch, so it steps every scan at 16 rows per lane
+        specs = make_specs(split=not plain)
         if plain:
             tri_engine.sweep_plain(specs)
         else:
@@ -153,7 +165,7 @@ def main() -> int:
     # the plain form captured once into a HIP graph (all 1023 x 8 step launches)
     graph_ms = None
     if not args.no_extras:
-        gspecs = make_specs()
+        gspecs = make_specs(split=False)
         g = tri_engine.PlainGraph(gspecs)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -407,7 +419,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_SWEEP_TRAFFIC.get((args.nz, args.ppb)),
+            "traffic": MEASURED_SWEEP_TRAFFIC.get((args.nz, args.ppb, args.split_chunks)),
             "launch_ms": t_kernel * 1e3,
             "algorithmic_bytes": alg_bytes,
         },

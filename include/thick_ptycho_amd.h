@@ -81,7 +81,7 @@
 extern "C" {
 #endif
 
-#define TP_ABI_VERSION 1
+#define TP_ABI_VERSION 2
 
 #define TP_OK 0
 #define TP_ERR_INVALID (-1)     /* bad argument or unsupported size      */
@@ -181,6 +181,12 @@ typedef struct tp_tri_scan {
    * of the object; persistent sweep only, with probes_per_block = 1 and no
    * gradient accumulation. */
   int64_t coef_pstride;
+  /* 0: `coef` was factored with the launch's chunk.  Otherwise the chunk it was
+   * factored with, which may be half the launch's (launch chunk 16 or 32, no
+   * gradient accumulation, n <= 64*chunk): a narrow scan then steps with half
+   * as many rows per lane beside the wide ones of the same persistent launch.
+   * The plain one-step form takes 0 only. */
+  int chunk;
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe

@@ -89,3 +89,18 @@ def test_new_entry_points_reject_bad_arguments():
     assert lib.tp_cnn_apply(1, 1, 1, 0, 8, 8, 1, None) == _lib.TP_ERR_INVALID
     assert lib.tp_cnn_param_count() == 9 * (16 + 256 + 256 + 16)
     assert lib.tp_band_coef_bytes(128, 128, 255, 6) == 255 * 128 * 14 * 128 * 8
+
+
+def test_sweep_rejects_unsupported_chunk_mix():
+    """A scan may take half the launch's chunk (16 or 32) only; anything else is
+    refused on the host before any device work."""
+    lib = _lib.load()
+    scans = (_lib.TriScan * 2)()
+    for s, (n, ch) in zip(scans, ((200, 4), (1000, 0))):
+        s.n, s.steps, s.probes, s.chunk = n, 10, 1, ch
+    rc = lib.tp_tri_sweep(scans, 2, 16, 4, None)
+    assert rc == _lib.TP_ERR_INVALID
+    assert b"half the launch's chunk" in lib.tp_last_error()
+    scans[0].chunk = 8
+    rc = lib.tp_tri_step(scans, 2, 16, 1, None)  # the plain form takes one chunk per launch
+    assert rc == _lib.TP_ERR_INVALID

@@ -240,3 +240,35 @@ def test_host_handover_in_chunks(gpu, rng, monkeypatch):
     assert h.dtype == np.complex128 and h.shape == (1, 1, 5, nx, nz)
     ref = dev.transpose(-1, -2).cpu().numpy().astype(np.complex128)
     np.testing.assert_array_equal(h, ref)
+
+
+def test_half_chunk_scans_in_one_launch(gpu):
+    """Config 1 with the 384-wide scan factored at 8 rows per lane and the others
+    at 16: one persistent launch of all eight batches equals the two chunk
+    widths launched apart (bitwise), and the narrow scan's exit planes match the
+    complex128 oracle."""
+    wl = workloads.config1()
+    specs, outs, refs = [], [], []
+    for mode in ("forward", "adjoint"):
+        for sc in wl.scans:
+            g = pwe.Grid(dimension=2, nx=sc.nx, nz=sc.nz, dx=sc.dx, dz=sc.dz, k=sc.k)
+            A, B = operators.step_operators_1d(sc.nx, sc.k, sc.dx, sc.dz, sc.bc)
+            C = torch.from_numpy(pwe.object_contribution(g, sc.n)).to("cuda")
+            fac = tri_engine.factor(C, A, B, mode, chunk=operators.chunk_for(sc.nx))
+            u0 = torch.from_numpy(sc.probes).to("cuda", torch.complex64)
+            specs.append((fac, u0))
+            outs.append((torch.empty_like(u0), torch.empty_like(u0)))
+            if sc.nx == 384:
+                cache = pwe.build_lu_cache(g, sc.bc, sc.n, mode)
+                refs.append((len(outs) - 1, pwe.solve_batched(g, cache, sc.probes, mode, keep="exit")))
+    assert sorted({f.chunk for f, _ in specs}) == [8, 16]
+    tri_engine.sweep([tri_engine.TriScanSpec(f, u0, exit=o[0]) for (f, u0), o in zip(specs, outs)],
+                     probes_per_block=4)
+    for width in (16, 8):
+        sel = [i for i, (f, _) in enumerate(specs) if f.chunk == width]
+        tri_engine.sweep([tri_engine.TriScanSpec(specs[i][0], specs[i][1], exit=outs[i][1]) for i in sel],
+                         probes_per_block=4)
+    for mixed, apart in outs:
+        torch.testing.assert_close(mixed, apart, rtol=0, atol=0)
+    for i, ref in refs:
+        assert rel_l2(outs[i][0].cpu().numpy(), ref) < TOL

@@ -81,6 +81,7 @@ class TriScan(ctypes.Structure):
         ("grad_zdir", c_int),
         ("grad_skip_row0", c_int),
         ("coef_pstride", c_int64),
+        ("chunk", c_int),
     ]
 
 
@@ -215,7 +216,7 @@ class LibraryMissing(RuntimeError):
     pass
 
 
-ABI_VERSION = 1  # TP_ABI_VERSION of include/thick_ptycho_amd.h
+ABI_VERSION = 2  # TP_ABI_VERSION of include/thick_ptycho_amd.h
 
 
 def load() -> ctypes.CDLL:

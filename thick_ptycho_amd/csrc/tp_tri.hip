@@ -546,7 +546,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // current step computes, and the plane computed in step s is written at the
 // start of step s+1 (before that fetch) so the stores drain behind the math.
 template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
-__global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
+__device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk, const int sc) {
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
   constexpr int NA = SIGMA ? 6 : 5;
@@ -557,8 +557,6 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   const float2* cf_l = reinterpret_cast<const float2*>(smem_v);
   constexpr int TILE = NP + NP / 8;  // per-wavefront store tile (float2)
 
-  const int blk = xcd_block(blockIdx.x, a.nblk);
-  const int sc = find_scan(a, blk);
   const tp_tri_scan& d = a.scans[sc];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -689,6 +687,28 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   }
 }
 
+// A workgroup's probes all belong to one scan, so the choice between the
+// launch's chunk and half of it (tp_tri_scan.chunk) is uniform per workgroup;
+// the LDS is sized for the launch's chunk and the narrow body uses a prefix.
+template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
+__global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
+  // Blocks in launch order: the dispatcher spreads consecutive workgroups over
+  // the XCDs and CUs, so the first workgroups of the launch (the first scans)
+  // get a CU each before any CU takes a second one.  Grouping each XCD's blocks
+  // into one contiguous range instead (coefficient reuse in one L2) measured
+  // 3-4 % slower on the config-1 launch (4.07-4.21 ms against 3.95-4.08 ms):
+  // it stacks two wide-scan workgroups on the same CUs.
+  const int blk = (int)blockIdx.x;
+  const int sc = find_scan(a, blk);
+  if constexpr (!GRAD && CH >= 16) {
+    if (a.scans[sc].chunk == CH / 2) {
+      tri_sweep_body<CH / 2, PPB, SIGMA, GRAD, STAGED>(a, blk, sc);
+      return;
+    }
+  }
+  tri_sweep_body<CH, PPB, SIGMA, GRAD, STAGED>(a, blk, sc);
+}
+
 // ------------------------------------------------------------------ plain one-step
 // One launch per step; plane s-1 is read from memory and plane s written, both
 // through the per-wavefront LDS tile so that global accesses are coalesced.
@@ -767,7 +787,8 @@ static int check_chunk(int chunk) {
   }
 }
 
-static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, SweepArgs& a, int& sigma) {
+static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, SweepArgs& a, int& sigma,
+                      bool allow_half_chunk) {
   if (nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
     return TP_ERR_INVALID;
@@ -777,8 +798,14 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
   sigma = scans[0].sigma;
   for (int i = 0; i < nscans; ++i) {
     const tp_tri_scan& d = scans[i];
-    if (d.n < 1 || d.n > 64 * chunk) {
-      set_error("scan %d: n = %d outside [1, 64*chunk = %d]", i, d.n, 64 * chunk);
+    const int sch = d.chunk == 0 ? chunk : d.chunk;
+    if (sch != chunk && !(allow_half_chunk && 2 * sch == chunk && chunk >= 16 && d.grad == nullptr)) {
+      set_error("scan %d: chunk %d with a launch chunk of %d: a scan may take half the launch's chunk (16 or 32) "
+                "in the persistent sweep without gradient accumulation", i, sch, chunk);
+      return TP_ERR_INVALID;
+    }
+    if (d.n < 1 || d.n > 64 * sch) {
+      set_error("scan %d: n = %d outside [1, 64*chunk = %d]", i, d.n, 64 * sch);
       return TP_ERR_INVALID;
     }
     if (d.steps < 1 || d.probes < 0 || !d.coef || !d.u0) {
@@ -993,7 +1020,7 @@ extern "C" int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int
   if (rc) return rc;
   SweepArgs a;
   int sigma = 0;
-  rc = build_args(scans, nscans, chunk, probes_per_block, a, sigma);
+  rc = build_args(scans, nscans, chunk, probes_per_block, a, sigma, true);
   if (rc) return rc;
   if (a.nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -1015,7 +1042,7 @@ extern "C" int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int 
   SweepArgs a;
   int sigma = 0;
   const int ppb = 4;
-  rc = build_args(scans, nscans, chunk, ppb, a, sigma);
+  rc = build_args(scans, nscans, chunk, ppb, a, sigma, false);
   if (rc) return rc;
   for (int i = 0; i < nscans; ++i) {
     if (!scans[i].out || s < 1 || s > scans[i].steps || scans[i].grad) {

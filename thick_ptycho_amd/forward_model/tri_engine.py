@@ -213,45 +213,66 @@ def step_slice(fac: TriFactors, first: int, count: int) -> TriFactors:
                       fac.status, fac.sigma, keep=[fac])
 
 
-def _structs(specs: Sequence[TriScanSpec]):
+def _structs(specs: Sequence[TriScanSpec], allow_half_chunk: bool = False):
     if not 1 <= len(specs) <= _lib.TP_MAX_SCANS:
         raise ValueError(f"between 1 and {_lib.TP_MAX_SCANS} scans per launch")
-    arr = (_lib.TriScan * len(specs))(*[s.struct() for s in specs])
     chunk = max(s.fac.chunk for s in specs)
+    structs = []
     for s in specs:
+        st = s.struct()
         if s.fac.chunk != chunk:
-            raise ValueError("all scans of one launch must be factored with the same chunk")
-    return arr, chunk
+            if not (allow_half_chunk and 2 * s.fac.chunk == chunk and chunk >= 16 and s.grad is None):
+                raise ValueError("all scans of one launch must be factored with the same chunk (the persistent "
+                                 "sweep also takes half a chunk of 16 or 32 for scans without gradients)")
+            st.chunk = s.fac.chunk
+        structs.append(st)
+    return (_lib.TriScan * len(specs))(*structs), chunk
 
 
 def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) -> None:
-    """Persistent batched sweep over all steps of all scans (one launch)."""
-    arr, chunk = _structs(specs)
+    """Persistent batched sweep over all steps of all scans (one launch).  Scans
+    factored with half the widest scan's chunk (16 or 32) step with that many
+    rows per lane inside the same launch."""
+    arr, chunk = _structs(specs, allow_half_chunk=True)
     if chunk >= 32:
         probes_per_block = min(probes_per_block, 2)  # LDS: 2048-row blocks
     lib = _lib.load()
     _lib.check(lib.tp_tri_sweep(arr, len(specs), chunk, probes_per_block, _lib.stream_handle(stream)), "tp_tri_sweep")
 
 
+def _chunk_groups(specs: Sequence[TriScanSpec]):
+    """(struct array, count, chunk) per chunk among ``specs``: the plain form
+    launches each chunk separately."""
+    groups = {}
+    for spec in specs:
+        groups.setdefault(spec.fac.chunk, []).append(spec)
+    out = []
+    for chunk in sorted(groups, reverse=True):
+        arr, _ = _structs(groups[chunk])
+        out.append((arr, len(groups[chunk]), chunk))
+    return out
+
+
 def step(specs: Sequence[TriScanSpec], s: int, stream=None) -> None:
-    """Plain form: one launch advancing every probe by step ``s``."""
-    arr, chunk = _structs(specs)
+    """Plain form: one launch per chunk width advancing every probe by step ``s``."""
     lib = _lib.load()
-    _lib.check(lib.tp_tri_step(arr, len(specs), chunk, s, _lib.stream_handle(stream)), "tp_tri_step")
+    sh = _lib.stream_handle(stream)
+    for arr, n, chunk in _chunk_groups(specs):
+        _lib.check(lib.tp_tri_step(arr, n, chunk, s, sh), "tp_tri_step")
 
 
 def sweep_plain(specs: Sequence[TriScanSpec], stream=None) -> None:
-    """Plain form over all steps: one launch per slice."""
+    """Plain form over all steps: one launch per slice (and chunk width)."""
     L = specs[0].fac.steps
     for spec in specs:
         if spec.fac.steps != L or spec.out is None:
             raise ValueError("plain sweeps need equal step counts and full-field outputs")
-    arr, chunk = _structs(specs)
+    groups = _chunk_groups(specs)
     lib = _lib.load()
     sh = _lib.stream_handle(stream)
-    n = len(specs)
     for s in range(1, L + 1):
-        _lib.check(lib.tp_tri_step(arr, n, chunk, s, sh), "tp_tri_step")
+        for arr, n, chunk in groups:
+            _lib.check(lib.tp_tri_step(arr, n, chunk, s, sh), "tp_tri_step")
 
 
 class PlainGraph:
