@@ -201,6 +201,32 @@ int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int probes_per
  * writes plane 0) and writing plane s.  Requires `out`. */
 int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int s, void* stream);
 
+/* Wide grids: each probe's rows are split over `waves` wavefronts (1..16) of
+ * one workgroup, 64 * waves lanes of `chunk` rows (chunk = 8 or 16), so
+ * n <= 64 * waves * chunk (16384 rows at chunk 16 and 16 wavefronts).  The
+ * coefficient buffer is complex64 [steps][6][n_pad] with n_pad = 64*waves*chunk
+ * and row i at (i % chunk) * 64 * waves + i / chunk (with waves = 1 the layout
+ * above); tp_tri_coef_bytes_wide gives its size.
+ * Replaces the same reference interfaces as tp_tri_factor / tp_tri_sweep
+ * (iterative_lu.py:66-197) for the widths of the reference's own examples
+ * (N = 12001 and 7400: notebooks/2D_examples/1_generate_simulated_data.ipynb,
+ * 4.pwe_parallel_in_time_forward.ipynb), which the one-wavefront kernels
+ * (n <= 2048) do not hold. */
+size_t tp_tri_coef_bytes_wide(int n, int steps, int chunk, int waves);
+
+/* tp_tri_factor for the wide layout: one workgroup of 64 * waves lanes per step. */
+int tp_tri_factor_wide(const void* d_C, int n, int ncol, const void* d_base,
+                       double lhs_sign, double rhs_sign, int conj_c,
+                       int lhs_col0, int rhs_col0, int col_step, int steps,
+                       int chunk, int waves, int with_sigma, void* d_coef, int* d_status,
+                       void* stream);
+
+/* Persistent sweep for the wide layout: one probe per workgroup; every field of
+ * tp_tri_scan applies (rhs, out/ckpt/exit, gradient accumulation with global
+ * atomics, per-probe factors through coef_pstride) except a half chunk: scans
+ * must leave `chunk` 0 or equal to the launch's. */
+int tp_tri_sweep_wide(const tp_tri_scan* scans, int nscans, int chunk, int waves, void* stream);
+
 /* ------------------------------------------------------------------------
  * Multislice (angular spectrum) and far field, n = 64 * r with r = 1 .. 32
  * ------------------------------------------------------------------------ */

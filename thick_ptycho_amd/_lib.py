@@ -148,6 +148,13 @@ SIGNATURES = {
     ),
     "tp_tri_sweep": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
     "tp_tri_step": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
+    "tp_tri_coef_bytes_wide": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "tp_tri_factor_wide": (
+        c_int,
+        [c_void_p, c_int, c_int, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+         c_int, c_void_p, c_void_p, c_void_p],
+    ),
+    "tp_tri_sweep_wide": (c_int, [ctypes.POINTER(TriScan), c_int, c_int, c_int, c_void_p]),
     "tp_ms_transmission": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p, c_void_p]),
     "tp_ms_kernel": (c_int, [c_int, c_double, c_double, c_double, c_void_p, c_void_p]),
     "tp_ms_sweep": (c_int, [ctypes.POINTER(MsScan), c_int, c_void_p]),

@@ -70,7 +70,7 @@ class PWEFullLUSolver(PWEIterativeLUSolver):
         if self.dimension == 3:
             fac = band_engine.factor(C, fm.A, fm.B, fmode)
         else:
-            fac = tri_engine.factor(C, fm.A, fm.B, fmode, chunk=self.chunk)
+            fac = self._tri_factor(C, fmode)
         cache = self.projection_cache[proj_idx].modes[mode]
         cache.factors = fac
         cache.b = np.zeros(self.block_size * (self.nz - 1), dtype=complex)

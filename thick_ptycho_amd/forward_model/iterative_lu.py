@@ -19,7 +19,7 @@ import torch
 
 from . import band_engine, tri_engine
 from .base_solver import BasePWESolver, host_complex128
-from .operators import BoundaryType, chunk_for, step_operators_1d
+from .operators import BoundaryType, layout_for, step_operators_1d
 
 
 class PWEForwardModel:
@@ -86,7 +86,8 @@ class PWEIterativeLUSolver(BasePWESolver):
         self.pwe_finite_differences = PWEForwardModel(simulation_space, bc_type=bc_type)
         self.probes_per_block = probes_per_block
         self.dimension = simulation_space.dimension
-        self.chunk = chunk_for(self.block_size) if self.dimension == 2 else None
+        # 2D stepping layout: rows per lane and wavefronts per probe (> 1 above 2048 rows)
+        self.chunk, self.waves = layout_for(self.block_size) if self.dimension == 2 else (None, None)
 
     # ------------------------------------------------------------------ factors
     def object_contribution_device(self, n, scan_idx: int = 0) -> torch.Tensor:
@@ -100,12 +101,17 @@ class PWEIterativeLUSolver(BasePWESolver):
         if self.dimension == 3:
             fac = band_engine.factor(C, fm.A, fm.B, mode)
         else:
-            fac = tri_engine.factor(C, fm.A, fm.B, mode, chunk=self.chunk)
+            fac = self._tri_factor(C, mode)
         cache = self.projection_cache[proj_idx].modes[mode]
         cache.factors = fac
         cache.b = np.zeros(self.block_size, dtype=complex) * fac.b_sign
         cache.cached_n = n
         self._timed(f"factored {self.nz - 1} slices ({mode})", t0)
+
+    def _tri_factor(self, C, mode: str, out=None):
+        fm = self.pwe_finite_differences
+        return tri_engine.factor(C, fm.A, fm.B, mode, chunk=self.chunk, waves=self.waves if self.waves > 1 else None,
+                                 out=out)
 
     # ------------------------------------------------------------------ rhs handling
     def _rhs_steps(self, mode: str, rhs_block, per_probe: bool) -> Optional[torch.Tensor]:
@@ -223,7 +229,8 @@ class PWEIterativeLUSolver(BasePWESolver):
         angle) steps every scan through its own set (tp_tri_scan.coef_pstride)."""
         fm = self.pwe_finite_differences
         S, L = self.num_probes, self.nz - 1
-        per = tri_engine.coef_elems(L, self.chunk)
+        per = tri_engine.coef_elems(L, self.chunk, self.waves)
+        wide = self.waves > 1
         t0 = time.time()
         windows = self._reduced_windows()
         ss = self.simulation_space
@@ -246,12 +253,14 @@ class PWEIterativeLUSolver(BasePWESolver):
             fac = None
             for i in range(0, S, 8):
                 jobs = [(contribution(s), fm.A, fm.B, mode) for s in range(i, min(i + 8, S))]
-                facs = tri_engine.factor_many(jobs, self.chunk, out=[coef[s] for s in range(i, min(i + 8, S))])
+                facs = tri_engine.factor_many(jobs, self.chunk, out=[coef[s] for s in range(i, min(i + 8, S))],
+                                              waves=self.waves if wide else None)
                 fac = fac or facs[0]
             whole = tri_engine.TriFactors(coef.reshape(-1), fac.n, L, self.chunk, mode, fac.b_sign, fac.status,
-                                          fac.sigma, keep=[coef])
+                                          fac.sigma, keep=[coef], waves=self.waves, wide=wide)
             if plain:  # the plain form shares factors within a scan: one scan per window
-                views = [tri_engine.TriFactors(coef[s], fac.n, L, self.chunk, mode, fac.b_sign, fac.status, fac.sigma)
+                views = [tri_engine.TriFactors(coef[s], fac.n, L, self.chunk, mode, fac.b_sign, fac.status, fac.sigma,
+                                               waves=self.waves, wide=wide)
                          for s in range(S)]
             for a in range(self.num_angles):
                 kw = {}

@@ -162,10 +162,35 @@ def mode_plan(A: Tridiag, B: Tridiag, mode: str, L: int) -> ModePlan:
 
 
 def chunk_for(n: int) -> int:
-    """Rows per GPU lane: the smallest power of two with 64 * chunk >= n."""
+    """Rows per GPU lane of the one-wavefront kernels: the smallest power of two
+    with 64 * chunk >= n (n <= 2048)."""
     for c in (1, 2, 4, 8, 16, 32):
         if 64 * c >= n:
             return c
     raise ValueError(
-        f"transverse size {n} exceeds the 2048 rows one wavefront holds; larger widths are not built yet"
+        f"transverse size {n} exceeds the 2048 rows one wavefront holds; use layout_for(n) (several wavefronts "
+        "per probe)"
     )
+
+
+WIDE_MAX_ROWS = 64 * 16 * 16  # 16 wavefronts of 64 lanes x 16 rows
+
+
+def layout_for(n: int, wide_chunk: int = 16) -> tuple:
+    """(chunk, waves) for the 2D stepping kernels: waves = 1 is one wavefront per
+    probe (n <= 2048, chunk from chunk_for); above that each probe's rows are
+    split over `waves` wavefronts of one workgroup with `wide_chunk` (8 or 16)
+    rows per lane, up to 16 wavefronts (16384 rows at chunk 16)."""
+    if n <= 64 * 32:
+        return chunk_for(n), 1
+    if wide_chunk not in (8, 16):
+        raise ValueError(f"wide_chunk must be 8 or 16 (got {wide_chunk})")
+    waves = -(-n // (64 * wide_chunk))
+    if waves > 16:
+        if wide_chunk == 8:
+            return layout_for(n, 16)
+        raise ValueError(
+            f"transverse size {n} exceeds the {WIDE_MAX_ROWS} rows of the widest 2D stepping layout "
+            "(16 wavefronts x 64 lanes x 16 rows)"
+        )
+    return wide_chunk, waves

@@ -50,7 +50,7 @@ from .. import _lib
 from . import tri_engine
 from .base_solver import BasePWESolver, host_complex128
 from .iterative_lu import PWEForwardModel
-from .operators import BoundaryType, Tridiag, chunk_for
+from .operators import BoundaryType, Tridiag, layout_for
 
 
 def pint_matvec(ops: torch.Tensor, Ct: torch.Tensor, x: torch.Tensor, b: Optional[torch.Tensor] = None,
@@ -82,7 +82,7 @@ class PiTPreconditioner:
 
     def __init__(self, A: Tridiag, B: Tridiag, N: int, L: int, alpha: float, device, chunk: Optional[int] = None):
         self.A, self.B, self.N, self.L, self.alpha = A, B, N, L, alpha
-        self.chunk = chunk or chunk_for(N)
+        self.chunk, self.waves = (chunk, 1) if chunk else layout_for(N)
         gamma = alpha ** (np.arange(L) / L)
         omega = np.exp(2j * np.pi * np.arange(L) / L) * alpha ** (1.0 / L)
         E = np.exp(2j * np.pi * np.outer(np.arange(L), np.arange(L)) / L) / np.sqrt(L)  # ifft(norm='ortho')
@@ -90,16 +90,18 @@ class PiTPreconditioner:
         self.Wb = torch.from_numpy(np.conj(E) / gamma[:, None]).to(device, torch.complex64)
         shift = 2 * omega / (1 + omega)
         Ccols = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(shift[None, :], (N, L)))).to(device)
-        self.fac = tri_engine.factor(Ccols, A, B, "forward", chunk=self.chunk)
+        self.fac = tri_engine.factor(Ccols, A, B, "forward", chunk=self.chunk,
+                                     waves=self.waves if self.waves > 1 else None)
         self.fac.check()
         f = self.fac
-        self._one = tri_engine.TriFactors(f.coef, N, 1, self.chunk, "forward", f.b_sign, f.status, False, keep=[f])
+        self._one = tri_engine.TriFactors(f.coef, N, 1, self.chunk, "forward", f.b_sign, f.status, False, keep=[f],
+                                          waves=f.waves, wide=f.wide)
         self._zero = torch.zeros((1, N), dtype=torch.complex64, device=device).expand(L, N)
 
     def block_solve(self, Xh: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         """out[p, k] = (A - (2 w_k / (1 + w_k)) I)^{-1} Xh[p, k] for every probe p and frequency k."""
         P = Xh.shape[0]
-        per = 6 * 64 * self.chunk
+        per = tri_engine.coef_elems(1, self.chunk, self.waves)
         specs = [tri_engine.TriScanSpec(self._one, self._zero, exit=out[p], rhs=Xh[p].unsqueeze(1), rhs_steps=1,
                                         coef_pstride=per) for p in range(P)]
         for i in range(0, P, _lib.TP_MAX_SCANS):

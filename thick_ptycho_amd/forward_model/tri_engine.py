@@ -5,6 +5,11 @@ per-step coefficients on the GPU; ``TriScanSpec`` describes one batch of
 probes stepped through those factors; ``sweep`` launches the persistent
 batched kernel over up to eight such batches (scans of different widths in
 one launch) and ``step`` the plain one-launch-per-slice form.
+
+Widths above 2048 rows take the wide layout (``operators.layout_for``): each
+probe's rows are split over several wavefronts of one workgroup
+(``tp_tri_factor_wide`` / ``tp_tri_sweep_wide``); ``TriFactors.wide`` marks
+factors in that layout.
 """
 
 from __future__ import annotations
@@ -16,7 +21,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .operators import ModePlan, Tridiag, chunk_for, mode_plan
+from .operators import ModePlan, Tridiag, layout_for, mode_plan
 
 
 @dataclass
@@ -30,6 +35,8 @@ class TriFactors:
     status: torch.Tensor
     sigma: bool
     keep: list = field(default_factory=list)
+    waves: int = 1  # wavefronts per probe (wide layout only)
+    wide: bool = False  # factored for tp_tri_sweep_wide (row i at (i % chunk) * 64 * waves + i / chunk)
 
     def check(self) -> None:
         """Raise if a pivot was exactly zero (synchronises)."""
@@ -60,39 +67,65 @@ def _plan_and_base(A: Tridiag, B: Tridiag, mode: str, L: int, dev):
 
 
 def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[int] = None,
-           stream=None) -> TriFactors:
+           stream=None, waves: Optional[int] = None, out: Optional[torch.Tensor] = None) -> TriFactors:
     """Factor every slice's LHS for ``mode``.
 
-    C: complex128 CUDA tensor (n, L) from create_object_contribution(...).reshape(-1, nz-1)."""
+    C: complex128 CUDA tensor (n, L) from create_object_contribution(...).reshape(-1, nz-1).
+    chunk / waves: the layout; by default ``layout_for(n)``.  ``waves`` given
+    (even 1) selects the wide kernels with that many wavefronts per probe.
+    out: optional complex64 buffer of coef_elems(L, chunk, waves) values."""
     assert C.is_cuda and C.dtype == torch.complex128 and C.dim() == 2
     n, L = C.shape
     C = C.contiguous()
-    chunk = chunk or chunk_for(n)
+    if chunk is None and waves is None:
+        chunk, nw = layout_for(n)
+        wide = nw > 1
+    else:
+        chunk = chunk or layout_for(n)[0]
+        wide = waves is not None
+        nw = waves if wide else 1
     dev = C.device
     plan, base = _plan_and_base(A, B, mode, L, dev)
-    coef = torch.empty(L * 6 * 64 * chunk, dtype=torch.complex64, device=dev)
+    elems = coef_elems(L, chunk, nw)
+    if out is not None:
+        assert out.dtype == torch.complex64 and out.is_contiguous() and out.numel() == elems
+        coef = out
+    else:
+        coef = torch.empty(elems, dtype=torch.complex64, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     lib = _lib.load()
     sh = _lib.stream_handle(stream)
-    _lib.check(
-        lib.tp_tri_factor(_lib.ptr(C), n, L, _lib.ptr(base), plan.lhs_sign, plan.rhs_sign, plan.conj_c,
-                          plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, int(plan.sigma), _lib.ptr(coef),
-                          _lib.ptr(status), sh),
-        "tp_tri_factor",
-    )
-    return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C])
+    if wide:
+        _lib.check(
+            lib.tp_tri_factor_wide(_lib.ptr(C), n, L, _lib.ptr(base), plan.lhs_sign, plan.rhs_sign, plan.conj_c,
+                                   plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, nw, int(plan.sigma),
+                                   _lib.ptr(coef), _lib.ptr(status), sh),
+            "tp_tri_factor_wide",
+        )
+    else:
+        _lib.check(
+            lib.tp_tri_factor(_lib.ptr(C), n, L, _lib.ptr(base), plan.lhs_sign, plan.rhs_sign, plan.conj_c,
+                              plan.lhs_col0, plan.rhs_col0, plan.col_step, L, chunk, int(plan.sigma), _lib.ptr(coef),
+                              _lib.ptr(status), sh),
+            "tp_tri_factor",
+        )
+    return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C], waves=nw, wide=wide)
 
 
-def coef_elems(L: int, chunk: int) -> int:
-    """Complex elements of one factor set of L slices (tp_tri_coef_bytes / 8)."""
-    return L * 6 * 64 * chunk
+def coef_elems(L: int, chunk: int, waves: int = 1) -> int:
+    """Complex elements of one factor set of L slices (tp_tri_coef_bytes[_wide] / 8)."""
+    return L * 6 * 64 * waves * chunk
 
 
-def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None) -> list:
+def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves: Optional[int] = None) -> list:
     """``factor`` for up to eight independent (C, A, B, mode) jobs in one launch
     (tp_tri_factor_batch); returns their TriFactors in order.  ``out`` optionally
-    gives each job's coefficient buffer (complex64, coef_elems(L, chunk) values)."""
+    gives each job's coefficient buffer (complex64, coef_elems(L, chunk, waves)
+    values).  ``waves`` selects the wide layout, factored one job per launch."""
     assert 1 <= len(jobs) <= _lib.TP_MAX_SCANS
+    if waves is not None:
+        return [factor(C, A, B, mode, chunk=chunk, waves=waves, stream=stream, out=None if out is None else out[k])
+                for k, (C, A, B, mode) in enumerate(jobs)]
     facs, arr = [], (_lib.TriFactorJob * len(jobs))()
     for k, (C, A, B, mode) in enumerate(jobs):
         assert C.is_cuda and C.dtype == torch.complex128 and C.dim() == 2
@@ -208,9 +241,9 @@ class TriScanSpec:
 def step_slice(fac: TriFactors, first: int, count: int) -> TriFactors:
     """View of ``fac`` restricted to steps first+1 .. first+count (no copy)."""
     assert 0 <= first and first + count <= fac.steps and count >= 1
-    per = 6 * 64 * fac.chunk
+    per = coef_elems(1, fac.chunk, fac.waves)
     return TriFactors(fac.coef[first * per:(first + count) * per], fac.n, count, fac.chunk, fac.mode, fac.b_sign,
-                      fac.status, fac.sigma, keep=[fac])
+                      fac.status, fac.sigma, keep=[fac], waves=fac.waves, wide=fac.wide)
 
 
 def _structs(specs: Sequence[TriScanSpec], allow_half_chunk: bool = False):
@@ -232,10 +265,25 @@ def _structs(specs: Sequence[TriScanSpec], allow_half_chunk: bool = False):
 def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) -> None:
     """Persistent batched sweep over all steps of all scans (one launch).  Scans
     factored with half the widest scan's chunk (16 or 32) step with that many
-    rows per lane inside the same launch."""
+    rows per lane inside the same launch.  Wide-layout scans go to
+    tp_tri_sweep_wide, one launch per (chunk, waves)."""
+    wide = [s for s in specs if s.fac.wide]
+    if wide:
+        groups = {}
+        for spec in wide:
+            groups.setdefault((spec.fac.chunk, spec.fac.waves), []).append(spec)
+        lib = _lib.load()
+        for (chunk, waves), group in groups.items():
+            arr = (_lib.TriScan * len(group))(*[g.struct() for g in group])
+            _lib.check(lib.tp_tri_sweep_wide(arr, len(group), chunk, waves, _lib.stream_handle(stream)),
+                       "tp_tri_sweep_wide")
+        specs = [s for s in specs if not s.fac.wide]
+        if not specs:
+            return
     arr, chunk = _structs(specs, allow_half_chunk=True)
     if chunk >= 32:
-        probes_per_block = min(probes_per_block, 2)  # LDS: 2048-row blocks
+        # LDS of 2048-row blocks: two probes per workgroup, one with gradient accumulation
+        probes_per_block = 1 if specs[0].grad is not None else min(probes_per_block, 2)
     lib = _lib.load()
     _lib.check(lib.tp_tri_sweep(arr, len(specs), chunk, probes_per_block, _lib.stream_handle(stream)), "tp_tri_sweep")
 
@@ -245,6 +293,9 @@ def _chunk_groups(specs: Sequence[TriScanSpec]):
     launches each chunk separately."""
     groups = {}
     for spec in specs:
+        if spec.fac.wide:
+            raise ValueError("the plain one-launch-per-slice form is built for the one-wavefront layout (n <= 2048); "
+                             "wide grids step in the persistent sweep")
         groups.setdefault(spec.fac.chunk, []).append(spec)
     out = []
     for chunk in sorted(groups, reverse=True):

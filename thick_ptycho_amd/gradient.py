@@ -169,18 +169,20 @@ def amplitude_gradient_reduced(n, probes, sqrt_i, windows, k: float, dx: float, 
         raise ValueError("windows must all have the same width and lie inside the object")
     A, B = _operators(W, k, dx, dz, bc)
     C = object_contribution(n_t, k, dz)  # (nx, L); a window's rows are its object term
-    chunk = operators.chunk_for(W)
-    per = tri_engine.coef_elems(L, chunk)
+    chunk, waves = operators.layout_for(W)
+    per = tri_engine.coef_elems(L, chunk, waves)
+    wide = waves > 1
     whole = {}
     for mode in ("forward", "exact_adjoint"):
         coef = torch.empty((S, per), dtype=torch.complex64, device=device)
         first = None
         for i in range(0, S, _lib.TP_MAX_SCANS):
             jobs = [(C[a:b], A, B, mode) for a, b in win[i:i + _lib.TP_MAX_SCANS]]
-            facs = tri_engine.factor_many(jobs, chunk, out=[coef[s] for s in range(i, min(i + _lib.TP_MAX_SCANS, S))])
+            facs = tri_engine.factor_many(jobs, chunk, out=[coef[s] for s in range(i, min(i + _lib.TP_MAX_SCANS, S))],
+                                          waves=waves if wide else None)
             first = first or facs[0]
         whole[mode] = tri_engine.TriFactors(coef.reshape(-1), W, L, chunk, mode, first.b_sign, first.status,
-                                            first.sigma, keep=[coef, facs])
+                                            first.sigma, keep=[coef, facs], waves=waves, wide=wide)
     u0 = _dev_tensor(probes, torch.complex64, device).reshape(S, W)
     U = torch.empty((S, L + 1, W), dtype=torch.complex64, device=device)
     tri_engine.sweep([tri_engine.TriScanSpec(whole["forward"], u0, out=U, coef_pstride=per)], probes_per_block=1)
