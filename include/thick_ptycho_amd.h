@@ -271,6 +271,48 @@ int tp_amplitude_seed(const void* d_exit, const void* d_sqrt_i, int n, int probe
 int tp_farfield(const void* d_exit, int64_t ld, int n, int probes, void* d_intensity, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Multislice, far field and amplitude seed at any length n (1 <= n <= 16384)
+ *
+ * Replace the same reference interfaces as tp_ms_sweep / tp_farfield /
+ * tp_amplitude_seed (ms_solver.py:49-159, base/base_solver.py:213-249,
+ * base_reconstructor.py:87-207), which numpy computes at any n (the
+ * reference's 2D reconstruction notebook uses n = 751).  Each operation is a
+ * circular convolution on F = tp_spectral_len(n) points (a power of two,
+ * F >= 2n - 1, at least 64), one workgroup per probe with the transform in
+ * LDS: multislice as the convolution with h = ifft_n(H), a length-n DFT by
+ * Bluestein's chirp-z identity.  For n > 8192 (F = 32768) the calls need a
+ * workspace of tp_spectral_work_bytes(n, probes) bytes; below it that size is 0.
+ * ------------------------------------------------------------------------ */
+int tp_spectral_len(int n);
+size_t tp_spectral_work_bytes(int n, int probes);
+/* Bytes of the FP64 set-up scratch the two table builders below take. */
+size_t tp_spectral_setup_bytes(int n);
+
+/* Multislice convolution spectrum S (complex64 [F], 1/F folded in) for the
+ * propagator H(kx) of tp_ms_kernel: S = FFT_F(h on (-n, n)) / F with
+ * h = ifft_n(H), all in FP64 direct sums.  d_setup: tp_spectral_setup_bytes(n). */
+int tp_ms_kernel_any(int n, double dx, double k, double dz, void* d_S, void* d_setup, void* stream);
+
+/* Bluestein tables: chirp w_m = exp(i pi m^2 / n) (complex64 [n]) and
+ * S = FFT_F(w on (-n, n)) / F (complex64 [F]). */
+int tp_chirp_tables(int n, void* d_chirp, void* d_S, void* d_setup, void* stream);
+
+/* tp_ms_sweep at any n: every scan shares n, and H points at the spectrum S of
+ * tp_ms_kernel_any (conj_h uses conj(S), the spectrum of ifft_n(conj(H))).
+ * One probe per workgroup; d_work: tp_spectral_work_bytes(n, total probes). */
+int tp_ms_sweep_any(const tp_ms_scan* scans, int nscans, void* d_work, void* stream);
+
+/* tp_farfield at any n, with the tables of tp_chirp_tables. */
+int tp_farfield_any(const void* d_exit, int64_t ld, int n, int probes, const void* d_chirp, const void* d_S,
+                    void* d_intensity, void* d_work, void* stream);
+
+/* tp_amplitude_seed at any n (the forward DFT and the unnormalised inverse by
+ * Bluestein), with the tables of tp_chirp_tables. */
+int tp_amplitude_seed_any(const void* d_exit, const void* d_sqrt_i, int n, int probes, double scale,
+                          const void* d_chirp, const void* d_S, void* d_seed, void* d_loss, void* d_work,
+                          void* stream);
+
+/* ------------------------------------------------------------------------
  * CN stepping with two transverse axes (N = nx * ny unknowns per slice)
  *
  * The per-slice operator is block tridiagonal: ny diagonal blocks A_jj

@@ -161,6 +161,15 @@ SIGNATURES = {
     "tp_ms_step": (c_int, [ctypes.POINTER(MsScan), c_int, c_int, c_void_p]),
     "tp_amplitude_seed": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "tp_farfield": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p]),
+    "tp_spectral_len": (c_int, [c_int]),
+    "tp_spectral_work_bytes": (c_size_t, [c_int, c_int]),
+    "tp_spectral_setup_bytes": (c_size_t, [c_int]),
+    "tp_ms_kernel_any": (c_int, [c_int, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+    "tp_chirp_tables": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tp_ms_sweep_any": (c_int, [ctypes.POINTER(MsScan), c_int, c_void_p, c_void_p]),
+    "tp_farfield_any": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tp_amplitude_seed_any": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p]),
     "tp_band_coef_bytes": (c_int64, [c_int, c_int, c_int, c_int]),
     "tp_band_factor": (
         c_int,

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtp_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["tp_status.cpp", "tp_tri.hip", "tp_triw.hip", "tp_ms.hip", "tp_band.hip", "tp_learn.hip", "tp_krylov.hip", "tp_ms2d.hip",
+SOURCES = ["tp_status.cpp", "tp_tri.hip", "tp_triw.hip", "tp_ms.hip", "tp_fftany.hip", "tp_band.hip", "tp_learn.hip", "tp_krylov.hip", "tp_ms2d.hip",
            "tp_pint.hip", "tp_pie.hip", "tp_window.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

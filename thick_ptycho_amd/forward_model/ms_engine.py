@@ -1,4 +1,12 @@
-"""Device-side plumbing for the multislice and far-field kernels."""
+"""Device-side plumbing for the multislice and far-field kernels.
+
+Widths n = 64 r (r <= 32) run the one-wavefront kernels (tp_ms_sweep,
+tp_farfield, tp_amplitude_seed); every other width up to 16384 runs the
+any-length forms (tp_ms_sweep_any, tp_farfield_any, tp_amplitude_seed_any), a
+circular convolution on a padded power-of-two grid: there the propagator is
+the convolution spectrum S of tp_ms_kernel_any in place of H / n, and the
+DFTs take Bluestein chirp tables, built once per width and device.
+"""
 
 from __future__ import annotations
 
@@ -8,6 +16,55 @@ from typing import Optional, Sequence
 import torch
 
 from .. import _lib
+
+
+def native(n: int) -> bool:
+    """True where the one-wavefront spectral kernels apply (n = 64 r, r <= 32)."""
+    return 64 <= n <= 2048 and n % 64 == 0
+
+
+def spectral_len(n: int) -> int:
+    """Padded convolution grid of the any-length forms (a power of two >= 2n - 1)."""
+    F = _lib.load().tp_spectral_len(int(n))
+    if F < 0:
+        raise ValueError(f"spectral length {n} outside [1, 16384]")
+    return F
+
+
+def propagator_len(n: int) -> int:
+    """Length of the propagator buffer the sweep takes at width n."""
+    return n if native(n) else spectral_len(n)
+
+
+def _setup_scratch(n: int, device) -> torch.Tensor:
+    nbytes = int(_lib.load().tp_spectral_setup_bytes(int(n)))
+    return torch.empty(max(nbytes, 16), dtype=torch.uint8, device=device)
+
+
+def _work(n: int, probes: int, device) -> Optional[torch.Tensor]:
+    nbytes = int(_lib.load().tp_spectral_work_bytes(int(n), int(probes)))
+    return torch.empty(nbytes, dtype=torch.uint8, device=device) if nbytes else None
+
+
+_CHIRP_CACHE: dict = {}
+
+
+def chirp_tables(n: int, device, stream=None):
+    """(chirp complex64 (n,), spectrum complex64 (F,)) for length-n DFTs, cached
+    per (n, device) (at most 16 entries)."""
+    key = (int(n), str(device))
+    hit = _CHIRP_CACHE.get(key)
+    if hit is None:
+        F = spectral_len(n)
+        chirp = torch.empty(n, dtype=torch.complex64, device=device)
+        S = torch.empty(F, dtype=torch.complex64, device=device)
+        setup = _setup_scratch(n, device)
+        _lib.check(_lib.load().tp_chirp_tables(int(n), _lib.ptr(chirp), _lib.ptr(S), _lib.ptr(setup),
+                                               _lib.stream_handle(stream)), "tp_chirp_tables")
+        if len(_CHIRP_CACHE) >= 16:
+            _CHIRP_CACHE.pop(next(iter(_CHIRP_CACHE)))
+        hit = _CHIRP_CACHE[key] = (chirp, S, setup)
+    return hit[0], hit[1]
 
 
 def transmission(n: torch.Tensor, k: float, dz: float, n_medium: float, stream=None) -> torch.Tensor:
@@ -25,7 +82,16 @@ def transmission(n: torch.Tensor, k: float, dz: float, n_medium: float, stream=N
 
 
 def propagator(nx: int, dx: float, k: float, dz: float, device, stream=None) -> torch.Tensor:
-    """H(kx) / nx in numpy.fft.fftfreq order (global phase removed, evanescent band clipped)."""
+    """H(kx) / nx in numpy.fft.fftfreq order (global phase removed, evanescent band
+    clipped); at widths the one-wavefront kernels do not take, the convolution
+    spectrum of the same propagator on spectral_len(nx) points."""
+    if not native(nx):
+        S = torch.empty(spectral_len(nx), dtype=torch.complex64, device=device)
+        setup = _setup_scratch(nx, device)
+        _lib.check(_lib.load().tp_ms_kernel_any(nx, float(dx), float(k), float(dz), _lib.ptr(S), _lib.ptr(setup),
+                                                _lib.stream_handle(stream)), "tp_ms_kernel_any")
+        S._tp_keep = setup  # the set-up scratch lives until the kernels that read it have run
+        return S
     H = torch.empty(nx, dtype=torch.complex64, device=device)
     _lib.check(_lib.load().tp_ms_kernel(nx, float(dx), float(k), float(dz), _lib.ptr(H), _lib.stream_handle(stream)),
                "tp_ms_kernel")
@@ -46,7 +112,7 @@ class MsScanSpec:
         L = self.T.shape[0]
         s = _lib.MsScan()
         assert self.T.dtype == torch.complex64 and self.T.shape[1] == n and self.T.is_contiguous()
-        assert self.H.dtype == torch.complex64 and self.H.shape == (n,)
+        assert self.H.dtype == torch.complex64 and self.H.shape == (propagator_len(n),)
         assert self.u0.dtype == torch.complex64 and self.u0.stride(-1) == 1
         s.T = _lib.ptr(self.T)
         s.H = _lib.ptr(self.H)
@@ -76,10 +142,19 @@ def _structs(specs: Sequence[MsScanSpec]):
 
 def sweep(specs: Sequence[MsScanSpec], stream=None) -> None:
     arr = _structs(specs)
-    _lib.check(_lib.load().tp_ms_sweep(arr, len(specs), _lib.stream_handle(stream)), "tp_ms_sweep")
+    n = specs[0].u0.shape[-1]
+    if native(n):
+        _lib.check(_lib.load().tp_ms_sweep(arr, len(specs), _lib.stream_handle(stream)), "tp_ms_sweep")
+        return
+    work = _work(n, sum(int(s.u0.shape[0]) for s in specs), specs[0].u0.device)
+    _lib.check(_lib.load().tp_ms_sweep_any(arr, len(specs), _lib.ptr(work), _lib.stream_handle(stream)),
+               "tp_ms_sweep_any")
 
 
 def sweep_plain(specs: Sequence[MsScanSpec], stream=None) -> None:
+    if not native(specs[0].u0.shape[-1]):
+        raise ValueError("the plain one-launch-per-slice multislice form is built for n = 64 r, r <= 32; other "
+                         "widths step in the persistent sweep")
     arr = _structs(specs)
     lib = _lib.load()
     sh = _lib.stream_handle(stream)
@@ -93,12 +168,37 @@ def farfield(exit_w: torch.Tensor, stream=None) -> torch.Tensor:
     assert exit_w.is_cuda and exit_w.dtype == torch.complex64 and exit_w.dim() == 2 and exit_w.stride(-1) == 1
     P, n = exit_w.shape
     out = torch.empty((P, n), dtype=torch.float32, device=exit_w.device)
-    _lib.check(
-        _lib.load().tp_farfield(_lib.ptr(exit_w), exit_w.stride(0) if P > 1 else n, n, P, _lib.ptr(out),
-                                _lib.stream_handle(stream)),
-        "tp_farfield",
-    )
+    ld = exit_w.stride(0) if P > 1 else n
+    if native(n):
+        _lib.check(_lib.load().tp_farfield(_lib.ptr(exit_w), ld, n, P, _lib.ptr(out), _lib.stream_handle(stream)),
+                   "tp_farfield")
+        return out
+    chirp, S = chirp_tables(n, exit_w.device, stream)
+    work = _work(n, P, exit_w.device)
+    _lib.check(_lib.load().tp_farfield_any(_lib.ptr(exit_w), ld, n, P, _lib.ptr(chirp), _lib.ptr(S), _lib.ptr(out),
+                                           _lib.ptr(work), _lib.stream_handle(stream)), "tp_farfield_any")
     return out
+
+
+def amplitude_seed(exit_w: torch.Tensor, sqrt_i: torch.Tensor, scale: float = 1.0, stream=None):
+    """(seed, loss per probe): seed = scale * dJ/d conj(exit) of J = sum (|fft(exit)| - sqrt_i)^2
+    (tp_amplitude_seed; tp_amplitude_seed_any at other widths)."""
+    assert exit_w.is_cuda and exit_w.dtype == torch.complex64 and exit_w.dim() == 2 and exit_w.is_contiguous()
+    assert sqrt_i.dtype == torch.float32 and sqrt_i.is_contiguous() and sqrt_i.shape == exit_w.shape
+    P, n = exit_w.shape
+    seed = torch.empty_like(exit_w)
+    loss = torch.empty(P, dtype=torch.float32, device=exit_w.device)
+    sh = _lib.stream_handle(stream)
+    if native(n):
+        _lib.check(_lib.load().tp_amplitude_seed(_lib.ptr(exit_w), _lib.ptr(sqrt_i), n, P, float(scale),
+                                                 _lib.ptr(seed), _lib.ptr(loss), sh), "tp_amplitude_seed")
+        return seed, loss
+    chirp, S = chirp_tables(n, exit_w.device, stream)
+    work = _work(n, P, exit_w.device)
+    _lib.check(_lib.load().tp_amplitude_seed_any(_lib.ptr(exit_w), _lib.ptr(sqrt_i), n, P, float(scale),
+                                                 _lib.ptr(chirp), _lib.ptr(S), _lib.ptr(seed), _lib.ptr(loss),
+                                                 _lib.ptr(work), sh), "tp_amplitude_seed_any")
+    return seed, loss
 
 
 # ---------------------------------------------------------------- two transverse axes

@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .forward_model import operators, tri_engine
+from .forward_model import ms_engine, operators, tri_engine
 
 
 def object_contribution(n: torch.Tensor, k: float, dz: float) -> torch.Tensor:
@@ -42,15 +42,7 @@ def object_contribution(n: torch.Tensor, k: float, dz: float) -> torch.Tensor:
 
 def amplitude_seed(exit_w: torch.Tensor, sqrt_i: torch.Tensor, scale: float = 1.0) -> Tuple[torch.Tensor, torch.Tensor]:
     """(seed, loss per probe): seed = scale * dJ/d conj(exit)."""
-    P, n = exit_w.shape
-    seed = torch.empty_like(exit_w)
-    loss = torch.empty(P, dtype=torch.float32, device=exit_w.device)
-    _lib.check(
-        _lib.load().tp_amplitude_seed(_lib.ptr(exit_w), _lib.ptr(sqrt_i), n, P, float(scale), _lib.ptr(seed),
-                                      _lib.ptr(loss), _lib.stream_handle()),
-        "tp_amplitude_seed",
-    )
-    return seed, loss
+    return ms_engine.amplitude_seed(exit_w.contiguous(), sqrt_i.contiguous(), scale)
 
 
 class _Sweeper:
