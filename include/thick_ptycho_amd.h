@@ -312,6 +312,23 @@ int tp_amplitude_seed_any(const void* d_exit, const void* d_sqrt_i, int n, int p
                           const void* d_chirp, const void* d_S, void* d_seed, void* d_loss, void* d_work,
                           void* stream);
 
+/* Strided batched DFT of any length n (Bluestein, tables of tp_chirp_tables):
+ * X_k = sum_j x_j exp(-2 pi i j k / n) (inverse != 0: exp(+...), unnormalised)
+ * for `count` sequences x_j = in[t*in_dist + j*in_stride] (complex64), written to
+ * out[t*out_dist + k*out_stride]: complex64, or |X_k|^2 as float32 with
+ * power != 0.  Strides in elements; d_work: tp_spectral_work_bytes(n, count).
+ * Builds the reference's two-axis far field (base/base_solver.py:237-242,
+ * numpy fft2 over the flattened (scans, N) exit waves) and the per-probe fft2
+ * of planes that are not powers of two. */
+int tp_dft_any(const void* d_in, int64_t in_stride, int64_t in_dist, void* d_out, int64_t out_stride,
+               int64_t out_dist, int n, int count, int inverse, int power, const void* d_chirp, const void* d_S,
+               void* d_work, void* stream);
+
+/* x[t*dist + k1*B + j2] *= exp(sign 2 pi i j2 k1 / (A B)) for t < count (FP64
+ * phases): the twiddles between the passes of a four-step DFT of length A*B
+ * (lengths above 16384, e.g. a 256 x 256 plane flattened). */
+int tp_twiddle_mul(void* d_x, int A, int B, int count, int64_t dist, int sign, void* stream);
+
 /* ------------------------------------------------------------------------
  * CN stepping with two transverse axes (N = nx * ny unknowns per slice)
  *

@@ -190,3 +190,40 @@ def test_3pie_at_odd_width(gpu):
     n_ref, h_ref = pie.reconstruct_3pie(probes, I, nx, nz, DX, K, DZ, max_iters=3, alpha_obj=0.1)
     assert np.allclose(h_gpu, h_ref, rtol=1e-4)
     assert rel_l2(n_gpu - 1.0, n_ref - 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("S,N", [(4, 480), (7, 751), (3, 16384), (5, 65536), (2, 30000)])
+def test_scan_axis_fft2_far_field(gpu, S, N, rng):
+    """The reference's two-axis far field: numpy fft2 over the flattened
+    (scans, N) exit waves (quirk Q9), including row lengths above 16384 (four-step)."""
+    ex = rng.standard_normal((S, N)) + 1j * rng.standard_normal((S, N))
+    got = ms_engine.farfield_scans_fft2(torch.from_numpy(ex).to("cuda", torch.complex64)).double().cpu().numpy()
+    ref = oms.farfield_intensities(ex, 3, S, N)
+    assert rel_l2(got, ref) < TOL
+
+
+@pytest.mark.parametrize("nx,ny", [(24, 20), (96, 80), (256, 256), (300, 200)])
+def test_per_probe_fft2_far_field(gpu, nx, ny, rng):
+    ex = rng.standard_normal((3, nx * ny)) + 1j * rng.standard_normal((3, nx * ny))
+    got = ms_engine.farfield_2d(torch.from_numpy(ex).to("cuda", torch.complex64), nx, ny).double().cpu().numpy()
+    ref = oms.farfield_per_probe_2d(ex, nx, ny).reshape(3, nx, ny)
+    assert rel_l2(got, ref) < TOL
+
+
+def test_drop_in_far_field_two_axes(gpu):
+    """get_farfield_intensities for two transverse axes through the drop-in: the
+    literal scan-axis fft2 by default and the per-probe form on request."""
+    nx, ny, nz, P = 24, 20, 8, 3
+    rng = np.random.default_rng(2)
+    n3 = workloads.ellipsoid_object_3d(nx, ny, nz, 3, rng, semi=(2.0, 6.0), delta=(0, 5e-3), beta=(0, 5e-5))
+    pr = workloads.gaussian_probes_2d(nx, ny, [(8, 7), (12, 10), (16, 13)], 4.0)
+    ss = SimulationSpace(nx, nz, 2.0, 4.0, ny=ny, dy=2.0, num_probes=P)
+    s = PWEIterativeLUSolver(ss, pr[None], bc_type="dirichlet")
+    u = s.solve(n=n3)
+    g3 = pwe.Grid(dimension=3, nx=nx, ny=ny, nz=nz, dx=2.0, dy=2.0, dz=4.0, k=K)
+    ref = pwe.solve(g3, "dirichlet", n3, pr.reshape(P, -1), "forward")  # (P, N, nz)
+    ex_ref = ref[:, :, -1]
+    ff = s.get_farfield_intensities(u, poisson_noise=False)
+    assert rel_l2(ff, oms.farfield_intensities(ex_ref, 3, P, nx * ny)) < TOL
+    ffp = s.get_farfield_intensities(u, poisson_noise=False, per_probe=True)
+    assert rel_l2(ffp, oms.farfield_per_probe_2d(ex_ref, nx, ny)) < TOL

@@ -170,6 +170,9 @@ SIGNATURES = {
     "tp_farfield_any": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "tp_amplitude_seed_any": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_void_p]),
+    "tp_dft_any": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_int64, c_int, c_int, c_int, c_int,
+                           c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tp_twiddle_mul": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int, c_void_p]),
     "tp_band_coef_bytes": (c_int64, [c_int, c_int, c_int, c_int]),
     "tp_band_factor": (
         c_int,

@@ -290,6 +290,65 @@ __global__ __launch_bounds__(1024) void fa_dft_kernel(const FaDftArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ strided DFT
+// X_k = sum_j x_j exp(-+2 pi i j k / n) of `count` sequences x_j = in[t*in_dist +
+// j*in_stride] into out[t*out_dist + k*out_stride] (complex64, or |X_k|^2 in
+// float32 with `power`); the inverse (unnormalised) as conj(DFT(conj(x))).
+struct FaStridedArgs {
+  const float2* in;
+  float2* out;
+  float* pout;
+  int64_t in_stride, in_dist, out_stride, out_dist;
+  const float2* chirp;
+  const float2* S;
+  int n, count, inverse, M, lg;
+  float2* work;
+};
+
+template <bool SPLIT>
+__global__ __launch_bounds__(1024) void fa_dft_strided_kernel(const FaStridedArgs a) {
+  extern __shared__ float2 fsm[];
+  const int M = a.M, lg = a.lg, T = blockDim.x, tid = threadIdx.x;
+  float2* buf = fsm;
+  float2* tab = fsm + M;
+  const int t = blockIdx.x;
+  fa_build_table(tab, M, tid, T);
+  __syncthreads();
+  float2* work = SPLIT ? a.work + (size_t)t * (2 * M) : nullptr;
+  const float2* x = a.in + t * a.in_dist;
+  const float2* w = a.chirp;
+  const bool inv = a.inverse != 0;
+  auto pre = [&](int j) {
+    const float2 v = x[j * a.in_stride];
+    return cmul(inv ? cconj(v) : v, cconj(w[j]));
+  };
+  auto post = [&](int k, float2 c) {
+    if (a.pout) {
+      a.pout[t * a.out_dist + k * a.out_stride] = fmaf(c.x, c.x, c.y * c.y);
+    } else {
+      const float2 X = cmul(cconj(w[k]), c);
+      a.out[t * a.out_dist + k * a.out_stride] = inv ? cconj(X) : X;
+    }
+  };
+  fa_convolve<SPLIT>(buf, tab, M, lg, a.n, a.S, false, work, pre, post, tid, T);
+}
+
+// x[t*dist + k1*B + j2] *= exp(sign 2 pi i j2 k1 / (A B)), the twiddles between
+// the two passes of a four-step DFT of length A B.
+__global__ void fa_twiddle_kernel(float2* __restrict__ x, int A, int B, int count, int64_t dist, int sign) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)A * B;
+  if (id >= per * count) return;
+  const int64_t t = id / per;
+  const int64_t r = id % per;
+  const int64_t k1 = r / B, j2 = r % B;
+  const int64_t m = (j2 * k1) % per;
+  double s, c;
+  sincospi(2.0 * (double)m / (double)per, &s, &c);
+  float2* p = x + t * dist + r;
+  *p = cmul(*p, make_float2((float)c, (float)(sign * s)));
+}
+
 // ------------------------------------------------------------------ set-up (FP64)
 // H_q = exp(i (kz_q - k) dz), kz = sqrt(max(k^2 - kx^2, 0)), kx = 2 pi fftfreq(n, dx)
 __global__ void fa_prop_kernel(int n, double dx, double k, double dz, double2* __restrict__ H) {
@@ -567,4 +626,61 @@ extern "C" int tp_amplitude_seed_any(const void* d_exit, const void* d_sqrt_i, i
   a.lg = lg;
   a.work = (float2*)d_work;
   return fa_dft_launch<1>(a, F, M, (hipStream_t)stream);
+}
+
+extern "C" int tp_dft_any(const void* d_in, int64_t in_stride, int64_t in_dist, void* d_out, int64_t out_stride,
+                          int64_t out_dist, int n, int count, int inverse, int power, const void* d_chirp,
+                          const void* d_S, void* d_work, void* stream) {
+  int F = 0, M = 0, lg = 0;
+  int rc = fa_grid(n, F, M, lg);
+  if (rc) return rc;
+  if (!d_in || !d_out || !d_chirp || !d_S || count < 0 || (F > M && !d_work)) {
+    set_error("tp_dft_any: invalid arguments (n=%d, count=%d; a workspace is needed above n = %d)", n, count,
+              FA_LDS_MAX / 2);
+    return TP_ERR_INVALID;
+  }
+  if (count == 0) return TP_OK;
+  FaStridedArgs a;
+  a.in = (const float2*)d_in;
+  a.out = power ? nullptr : (float2*)d_out;
+  a.pout = power ? (float*)d_out : nullptr;
+  a.in_stride = in_stride;
+  a.in_dist = in_dist;
+  a.out_stride = out_stride;
+  a.out_dist = out_dist;
+  a.chirp = (const float2*)d_chirp;
+  a.S = (const float2*)d_S;
+  a.n = n;
+  a.count = count;
+  a.inverse = inverse;
+  a.M = M;
+  a.lg = lg;
+  a.work = (float2*)d_work;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = fa_lds(M);
+  const int T = fa_threads(M);
+  if (F > M) {
+    rc = fa_set_lds(fa_dft_strided_kernel<true>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL((fa_dft_strided_kernel<true>), dim3(count), dim3(T), lds, st, a);
+  } else {
+    rc = fa_set_lds(fa_dft_strided_kernel<false>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL((fa_dft_strided_kernel<false>), dim3(count), dim3(T), lds, st, a);
+  }
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
+}
+
+extern "C" int tp_twiddle_mul(void* d_x, int A, int B, int count, int64_t dist, int sign, void* stream) {
+  if (!d_x || A < 1 || B < 1 || count < 0 || (sign != 1 && sign != -1) || dist < (int64_t)A * B) {
+    set_error("tp_twiddle_mul: invalid arguments (A=%d, B=%d, count=%d, sign=%d)", A, B, count, sign);
+    return TP_ERR_INVALID;
+  }
+  const int64_t total = (int64_t)A * B * count;
+  if (total == 0) return TP_OK;
+  hipLaunchKernelGGL(fa_twiddle_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (float2*)d_x, A, B, count, dist, sign);
+  TP_CHECK_HIP(hipGetLastError());
+  return TP_OK;
 }

@@ -115,14 +115,23 @@ class BaseForwardModelSolver(ABC):
         return u[..., -1].reshape((self.simulation_space.total_scans, self.simulation_space.block_size))
 
     def get_farfield_intensities(self, u=None, exit_waves=None, poisson_noise: Optional[bool] = True,
-                                 rng: Optional[np.random.Generator] = None) -> np.ndarray:
-        """|fft(exit)|^2 on the GPU (base/base_solver.py:213-249); Poisson noise on the host."""
+                                 rng: Optional[np.random.Generator] = None, per_probe: bool = False) -> np.ndarray:
+        """|fft(exit)|^2 on the GPU (base/base_solver.py:213-249); Poisson noise on the host.
+
+        Two transverse axes: by default the reference's literal fft2 over the
+        (total_scans, block_size) array, across the scan axis (quirk Q9);
+        ``per_probe=True`` gives each probe's |fft2|^2 over its (nx, ny) plane."""
         exit_waves = self.get_exit_waves(u=u) if exit_waves is None else exit_waves
-        if self.simulation_space.dimension != 2:
-            raise NotImplementedError("far field for two transverse axes is not built in thick_ptycho_amd")
         ew = exit_waves if isinstance(exit_waves, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(exit_waves))
         ew = ew.to(self.device, torch.complex64).reshape(-1, ew.shape[-1]).contiguous()
-        inten = ms_engine.farfield(ew).double().cpu().numpy()
+        if self.simulation_space.dimension == 2:
+            inten = ms_engine.farfield(ew)
+        elif per_probe:
+            nx, ny = self.effective_shape[0], self.effective_shape[1]
+            inten = ms_engine.farfield_2d(ew, nx, ny).reshape(ew.shape)
+        else:
+            inten = ms_engine.farfield_scans_fft2(ew)
+        inten = inten.double().cpu().numpy()
         if poisson_noise:
             inten = (rng.poisson(inten) if rng is not None else np.random.poisson(inten))
         return inten.reshape((self.simulation_space.total_scans, self.simulation_space.block_size))

@@ -201,6 +201,62 @@ def amplitude_seed(exit_w: torch.Tensor, sqrt_i: torch.Tensor, scale: float = 1.
     return seed, loss
 
 
+# ---------------------------------------------------------------- DFTs of any length
+def _dft(inp, in_stride, in_dist, out, out_stride, out_dist, n, count, inverse=False, power=False, stream=None):
+    """One tp_dft_any launch (n <= 16384)."""
+    chirp, S = chirp_tables(n, inp.device, stream)
+    work = _work(n, count, inp.device)
+    _lib.check(_lib.load().tp_dft_any(_lib.ptr(inp), int(in_stride), int(in_dist), _lib.ptr(out), int(out_stride),
+                                      int(out_dist), int(n), int(count), int(inverse), int(power), _lib.ptr(chirp),
+                                      _lib.ptr(S), _lib.ptr(work), _lib.stream_handle(stream)), "tp_dft_any")
+
+
+def _split(n: int):
+    """n = A * B with A, B <= 16384 and A closest to sqrt(n) (four-step DFT)."""
+    best = None
+    for a in range(1, int(n ** 0.5) + 1):
+        if n % a == 0 and n // a <= 16384:
+            best = a
+    if best is None:
+        raise ValueError(f"DFT length {n} above 16384 needs a factorisation into two lengths <= 16384")
+    return best, n // best
+
+
+def dft_rows(x: torch.Tensor, inverse: bool = False, power: bool = False, stream=None) -> torch.Tensor:
+    """DFT of every row of a contiguous complex64 (R, n) tensor on the GPU
+    (numpy.fft.fft convention; inverse unnormalised).  power=True returns |X|^2
+    as float32.  Rows above 16384 points take a four-step split n = A B."""
+    assert x.is_cuda and x.dtype == torch.complex64 and x.dim() == 2 and x.is_contiguous()
+    R, n = x.shape
+    out = torch.empty((R, n), dtype=torch.float32 if power else torch.complex64, device=x.device)
+    if n <= 16384:
+        _dft(x, 1, n, out, 1, n, n, R, inverse, power, stream)
+        return out
+    A, B = _split(n)
+    tmp = torch.empty((A, B), dtype=torch.complex64, device=x.device)
+    sign = 1 if inverse else -1
+    for r in range(R):
+        _dft(x[r], B, 1, tmp, B, 1, A, B, inverse, False, stream)  # over j1 of x[j2 + B j1]
+        _lib.check(_lib.load().tp_twiddle_mul(_lib.ptr(tmp), A, B, 1, A * B, sign, _lib.stream_handle(stream)),
+                   "tp_twiddle_mul")
+        _dft(tmp, 1, B, out[r], A, 1, B, A, inverse, power, stream)  # X[k1 + A k2]
+    return out
+
+
+def farfield_scans_fft2(exit_w: torch.Tensor, stream=None) -> torch.Tensor:
+    """|fft2|^2 over both axes of the (scans, N) exit-wave array, float32: the
+    reference's far field for two transverse axes, which applies numpy's fft2 to
+    the flattened (total_scans, block_size) array, i.e. across the scan axis
+    (base/base_solver.py:237-242)."""
+    assert exit_w.dim() == 2
+    ex = exit_w.to(torch.complex64).contiguous()
+    S, N = ex.shape
+    rows = dft_rows(ex, stream=stream)
+    out = torch.empty((S, N), dtype=torch.float32, device=ex.device)
+    _dft(rows, N, 1, out, N, 1, S, N, False, True, stream)  # along the scan axis, |X|^2
+    return out
+
+
 # ---------------------------------------------------------------- two transverse axes
 def propagator_2d(nx: int, ny: int, dx: float, dy: float, k: float, dz: float, device, stream=None) -> torch.Tensor:
     """H(kx, ky) / (nx ny) in numpy.fft.fftfreq order along both axes, stored transposed:
@@ -238,10 +294,19 @@ def multislice_2d(n: torch.Tensor, probes: torch.Tensor, dx: float, dy: float, k
 
 
 def farfield_2d(exit_w: torch.Tensor, nx: int, ny: int, stream=None) -> torch.Tensor:
-    """|fft2(exit)|^2 per probe, float32 (P, nx, ny); exit complex64 (P, nx*ny) or (P, nx, ny)."""
+    """|fft2(exit)|^2 per probe, float32 (P, nx, ny); exit complex64 (P, nx*ny) or (P, nx, ny).
+
+    Planes of powers of two from 64 to 512 take tp_farfield2d, others the
+    any-length DFT along y then x."""
     ex = exit_w.reshape(-1, nx * ny).to(torch.complex64).contiguous()
     P = ex.shape[0]
     out = torch.empty((P, nx, ny), dtype=torch.float32, device=ex.device)
+    m2 = (64, 128, 256, 512)
+    if nx not in m2 or ny not in m2:
+        rows = dft_rows(ex.reshape(P * nx, ny), stream=stream).reshape(P, nx * ny)
+        for p in range(P):  # along x: stride ny, one sequence per ky
+            _dft(rows[p], ny, 1, out[p], ny, 1, nx, ny, False, True, stream)
+        return out
     work = torch.empty((P, nx * ny), dtype=torch.complex64, device=ex.device)
     _lib.check(_lib.load().tp_farfield2d(_lib.ptr(ex), nx, ny, P, _lib.ptr(out), _lib.ptr(work),
                                          _lib.stream_handle(stream)), "tp_farfield2d")
