@@ -330,6 +330,28 @@ int tp_dft_any(const void* d_in, int64_t in_stride, int64_t in_dist, void* d_out
 int tp_twiddle_mul(void* d_x, int A, int B, int count, int64_t dist, int sign, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Object filters of the one-axis reconstruction loop
+ * (ReconstructorPWE.tv_denoise_amp / gaussian_lowpass_phase /
+ * global_object_update, thick_ptycho/reconstruction/pwe_reconstructor.py:370-419).
+ * Arrays are row-major (ny, nx); for the (x, z) object ny = x, nx = z.
+ * ------------------------------------------------------------------------ */
+
+/* A = |n|, Phi = arg(n) (float64) of a complex128 object. */
+int tp_object_polar(const void* d_n, int ny, int nx, void* d_A, void* d_Phi, void* stream);
+
+/* n = A exp(i Phi), complex128. */
+int tp_object_join(const void* d_A, const void* d_Phi, int ny, int nx, void* d_n, void* stream);
+
+/* In place: n_iter Chambolle-type TV iterations on the float64 array u with
+ * periodic differences and tau = 1/4 (lam > 0), then u - lam div(p).
+ * d_work: 3 * ny * nx float64. */
+int tp_tv_denoise(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* d_work, void* stream);
+
+/* In place on a complex64 spectrum X (ny, nx) in FFT order: X *= fftshift(W),
+ * W = exp(-(qx^2 + qy^2) / (2 sigma^2)), q = linspace(-1/2, 1/2, n, endpoint=False). */
+int tp_lowpass_weight(void* d_X, int ny, int nx, double sigma, void* stream);
+
+/* ------------------------------------------------------------------------
  * CN stepping with two transverse axes (N = nx * ny unknowns per slice)
  *
  * The per-slice operator is block tridiagonal: ny diagonal blocks A_jj

@@ -48,28 +48,39 @@ def to_vector_form(u: np.ndarray) -> np.ndarray:
     return u[:, :, 1:].transpose(0, 2, 1).reshape(P, -1)
 
 
-def reference_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_meas: np.ndarray):
-    """Return (grad_real, grad_imag), each (L*N,) step-major, as the reference."""
+def reference_gradient_from_states(g: pwe.Grid, adj: pwe.LUCache, uk: np.ndarray, err_last: np.ndarray, grad_A):
+    """compute_grad_least_squares (pwe_reconstructor.py:100-142) for one projection
+    and angle: the reference adjoint of each probe's exit-wave error (last column
+    of rhs_block), g = grad_A @ u, grad_re -= Re(conj(g) back), grad_im -=
+    Re(conj(i g) back).  uk: (P, L*N) vector form; err_last: (P, N)."""
     N = g.block_size
     L = g.nz - 1
+    grad_re = np.zeros(N * L)
+    grad_im = np.zeros(N * L)
+    for p in range(uk.shape[0]):
+        rhs = np.zeros((N, L), dtype=complex)
+        rhs[:, -1] = err_last[p]
+        back = pwe.solve_single_probe(g, adj, None, "adjoint", rhs_block=rhs)
+        back = back[:, :-1].T.ravel()
+        gb = grad_A @ uk[p]
+        grad_re -= (gb.conj() * back).real
+        grad_im -= ((1j * gb).conj() * back).real
+    return grad_re, grad_im
+
+
+def reference_gradient(g: pwe.Grid, bc_type, n: np.ndarray, probes: np.ndarray, I_meas: np.ndarray):
+    """Return (grad_real, grad_imag), each (L*N,) step-major, as the reference's
+    reconstruction loop computes them: compute_grad_least_squares is called with
+    grad_Ak = -get_gradient(n) (pwe_reconstructor.py:257, 281)."""
+    N = g.block_size
     P = probes.shape[0]
     fwd = pwe.build_lu_cache(g, bc_type, n, "forward")
     adj = pwe.build_lu_cache(g, bc_type, n, "adjoint")
     u = pwe.solve_batched(g, fwd, probes.reshape(P, N), "forward")
     uk = to_vector_form(u)
     exit_w = u[:, :, -1]
-    err = np.zeros((P, N, L), dtype=complex)
-    err[:, :, -1] = exit_w - phase_retrieval_model(exit_w, I_meas)
-    grad_A = gradient_operator(g, n)
-    grad_re = np.zeros(N * L)
-    grad_im = np.zeros(N * L)
-    for p in range(P):
-        back = pwe.solve_single_probe(g, adj, None, "adjoint", rhs_block=err[p])
-        back = back[:, :-1].T.ravel()
-        gb = grad_A @ uk[p]
-        grad_re -= (gb.conj() * back).real
-        grad_im -= ((1j * gb).conj() * back).real
-    return grad_re, grad_im
+    err_last = exit_w - phase_retrieval_model(exit_w, I_meas)
+    return reference_gradient_from_states(g, adj, uk, err_last, -gradient_operator(g, n))
 
 
 def amplitude_loss(exit_w: np.ndarray, I_meas: np.ndarray) -> float:

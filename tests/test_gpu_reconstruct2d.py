@@ -70,13 +70,13 @@ def test_reference_named_entry_point(gpu):
     g, n_true, n0, probes, I = _problem(seed=5)
     ss = SimulationSpace(256, 32, 2.0, 4.0, wavelength=1.0, num_probes=probes.shape[0])
     rc = reconstruct.ReconstructorPWE(ss, I, probes, bc_type="dirichlet")
-    n_est, wf, res = rc.reconstruct(max_iters=3, n_initial=n0)
+    n_est, wf, res = rc.reconstruct(max_iters=3, n_initial=n0, method="lipschitz")
     assert n_est.shape == (256, 32) and wf.shape == (1, probes.shape[0], 256, 31) and 2 <= len(res) <= 4
     u = pwe.solve_batched(g, pwe.build_lu_cache(g, "dirichlet", n0, "forward"), probes, keep="exit")
     e = u - og.phase_retrieval_model(u, I)
     assert res[0] == pytest.approx(np.sqrt(np.mean(np.abs(e) ** 2)), rel=1e-4)
     assert res[-1] < res[0]
-    _, _, res2 = rc.reconstruct(max_iters=3, n_initial=n0, tol=10 * res[0])
+    _, _, res2 = rc.reconstruct(max_iters=3, n_initial=n0, tol=10 * res[0], method="lipschitz")
     assert len(res2) == 1
 
 
@@ -104,7 +104,7 @@ def test_exit_wave_data(gpu):
     assert out["rmse_history"][0] == pytest.approx(np.sqrt(np.mean(np.abs(u0 - d) ** 2)), rel=1e-4)
     ss = SimulationSpace(256, 32, 2.0, 4.0, wavelength=1.0, num_probes=probes.shape[0])
     _, _, res = reconstruct.ReconstructorPWE(ss, d, probes, phase_retrieval=False, bc_type="dirichlet").reconstruct(
-        max_iters=2, n_initial=n0)
+        max_iters=2, n_initial=n0, method="lipschitz")
     assert res[-1] < res[0]
 
 

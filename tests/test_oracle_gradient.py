@@ -62,10 +62,11 @@ def test_reference_gradient_structure():
     ex = pwe.solve_batched(g, fwd, probes, keep="exit")
     e = ex - og.phase_retrieval_model(ex, I)
     assert np.allclose(e, og.amplitude_loss_seed(ex, I) / N)
-    # the reference gradient equals the closed form with its own adjoint states
+    # the reference gradient equals the closed form with its own adjoint states;
+    # the loop passes grad_Ak = -get_gradient(n), so g = -g0 (u_s + u_{s-1})
     adj = pwe.build_lu_cache(g, "dirichlet", n0, "adjoint")
     u = pwe.solve_batched(g, fwd, probes, "forward")
-    g0 = g.dz * g.k / 2j
+    g0 = -g.dz * g.k / 2j
     gr2 = np.zeros((L, N))
     gi2 = np.zeros((L, N))
     for p in range(probes.shape[0]):
@@ -148,7 +149,8 @@ def test_reference_gradient_equals_exact_adjoint_form_when_uniform_in_z(rng):
     """Pin of the restated compute_grad_least_squares: when the object does not
     vary along z the reference-form adjoint (quirk Q3: slice L+1-j's object term
     on both sides) coincides with the exact adjoint, so the reference gradient
-    must equal -conj(eo) sum_p conj(u_s + u_{s-1}) lambda_s / N with lambda from
+    (grad_Ak = -get_gradient(n), pwe_reconstructor.py:257) must equal
+    conj(eo) sum_p conj(u_s + u_{s-1}) lambda_s / N with lambda from
     the exact adjoint (the path pinned by finite differences above), eo =
     (dz/2)(k/i), the u_0 term absent at s = 1 and the 1/N of its exit-wave
     error.  A wrong flip, sign, conjugate or seed scale fails it; with an object
@@ -165,7 +167,7 @@ def test_reference_gradient_equals_exact_adjoint_form_when_uniform_in_z(rng):
         ref = (gr + 1j * gi).reshape(nz - 1, nx).T
         u = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", n, "forward"), probes, "forward")
         lam = og.exact_adjoint_states(g, "impedance", n, og.amplitude_loss_seed(u[:, :, -1], I))
-        pred = np.stack([-np.conj(eo) * np.sum(np.conj(u[:, :, s] + (u[:, :, s - 1] if s > 1 else 0))
+        pred = np.stack([np.conj(eo) * np.sum(np.conj(u[:, :, s] + (u[:, :, s - 1] if s > 1 else 0))
                                                 * lam[:, :, s], axis=0) / nx for s in range(1, nz)], axis=1)
         return ref, pred
 

@@ -14,7 +14,8 @@ adjoint kernel.  Memory: (L/checkpoint + 1) planes per probe instead of L + 1.
 (thick_ptycho/reconstruction/pwe_reconstructor.py:100-164): the reference
 adjoint (quirk Q3), the exit-wave error of base_reconstructor.py:87-207 (the
 seed scaled by 1/N), g = g0 (u_s + u_{s-1}) without the u_0 term, and the
-(grad_re, grad_im) split with the reference's signs.
+(grad_re, grad_im) split with the reference's signs (its loop passes
+grad_Ak = -get_gradient(n), pwe_reconstructor.py:257, 281).
 
 ``ExitWaves`` is a torch.autograd.Function mapping n -> exit waves whose
 backward runs the same checkpointed exact adjoint.
@@ -224,9 +225,17 @@ def reference_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="di
         amp = _dev_tensor(sqrt_i, torch.float32, device).reshape(u0.shape[0], N)
         err, _ = amplitude_seed(ex, amp, 1.0 / N)
     G = sw.adjoint(ck, err, skip_row0=True).to(torch.complex128)
-    g0 = dz * k / 2j
-    X = (g0 * G).reshape(-1)
-    return (-X.real).cpu().numpy(), X.imag.cpu().numpy()
+    return tuple(t.cpu().numpy() for t in reference_gradient_parts(G, k, dz))
+
+
+def reference_gradient_parts(G: torch.Tensor, k: float, dz: float):
+    """(grad_real, grad_imag) float64 (L*N,) step-major from the accumulation
+    G[s-1] = sum_p conj(lambda_s)(u_s + u_{s-1}) (no u_0 term) of the reference
+    adjoint states: with g = grad_Ak u = -g0 (u_s + u_{s-1}), g0 = dz k / 2i,
+    grad_re = -sum Re(conj(g) lambda) = Re(g0 G) and grad_im = -sum
+    Re(conj(i g) lambda) = -Im(g0 G)."""
+    X = ((dz * k / 2j) * G.to(torch.complex128)).reshape(-1)
+    return X.real.contiguous(), (-X.imag).contiguous()
 
 
 class ExitWaves(torch.autograd.Function):

@@ -8,17 +8,18 @@ Data files (numpy ``.npz``, the keys the reference writes):
 * reconstructions, ``np.savez_compressed(..., n_est=..., wave_field=...,
   residual_history=...)`` (reconstruct_sample.py:105-108).
 
-The driver minimises J(n) = sum_p sum (|fft(u_L^p)| - sqrt(I^p))^2 with the
-exact gradient of ``gradient.amplitude_gradient`` (checkpointed forward and
-exact-adjoint sweeps on the GPU).  ``method="nlcg"`` is the nonlinear conjugate
-gradient iteration of the reference's ReconstructorPWE.reconstruct
-(pwe_reconstructor.py:230-368): separate directions for Re n and Im n, each with
-beta = min(beta_FR, max(beta_PR, 0)) (compute_betak, pwe_reconstructor.py:221-228).
-The step differs: 0.5 / L with L a secant estimate of the gradient's Lipschitz
-constant instead of the reference's linearised line search (compute_alphak,
-which needs a forward solve per probe and direction), halved and restarted
-along -grad whenever an iterate raises J.  ``method="sd"`` is steepest descent
-with the same step.
+``ReferenceNLCG`` is the reference's ReconstructorPWE.reconstruct loop
+(pwe_reconstructor.py:230-368) on the GPU: its reference-form gradient, the
+linearised step of compute_alphak (one extra forward sweep of every probe per
+part), beta = min(beta_FR, max(beta_PR, 0)) per part (compute_betak,
+:221-228) and the TV / low-pass object update (:370-419), whose kernels are
+``tv_denoise_amp``, ``gaussian_lowpass_phase`` and ``global_object_update``
+here.  ``Reconstruction2D`` minimises J(n) = sum_p sum (|fft(u_L^p)| -
+sqrt(I^p))^2 with the exact gradient of ``gradient.amplitude_gradient``
+(checkpointed forward and exact-adjoint sweeps); ``method="nlcg"`` uses the same
+beta rule with the step 0.5 / L, L a secant estimate of the gradient's
+Lipschitz constant, halved and restarted along -grad whenever an iterate
+raises J; ``method="sd"`` is steepest descent with the same step.
 """
 
 from __future__ import annotations
@@ -123,12 +124,13 @@ class Reconstruction2D:
         return float(np.sqrt(max(J, 0.0) / (P * N * (N if self.phase_retrieval else 1))))
 
     def run(self, n0, iters: int, method: str = "nlcg", alpha: Optional[float] = None,
-            wave_field: bool = True, tol: Optional[float] = None) -> dict:
+            wave_field: bool = True, tol: Optional[float] = None, post=None) -> dict:
         """Returns {'n_est': complex128 (nx, nz), 'wave_field': ``wave_field(n_est)``,
         'residual_history': [J before the first iteration and after each accepted
         one], 'rmse_history': the same as exit-wave RMSE, 'alpha': the last step};
         the three arrays are what ``save_reconstruction`` writes.  With ``tol`` the
-        loop stops once the RMSE is below it (the reference's test)."""
+        loop stops once the RMSE is below it (the reference's test).  ``post``
+        (n -> n) is applied to every accepted iterate (the object filters)."""
         if method not in ("nlcg", "sd"):
             raise ValueError("method must be 'nlcg' or 'sd'")
         n = torch.as_tensor(np.asarray(n0)).to(self.dev, torch.complex128).clone()
@@ -152,6 +154,9 @@ class Reconstruction2D:
                 d = -g_new + torch.complex(b_re * d.real, b_im * d.imag)
             else:
                 d = -g_new
+            if post is not None:
+                n_new = post(n_new)
+                J_new, g_new = self.gradient(n_new)
             n, J, g = n_new, J_new, g_new
             hist.append(J)
         n_est = n.cpu().numpy()
@@ -159,15 +164,193 @@ class Reconstruction2D:
                 "residual_history": hist, "rmse_history": [self.rmse(v) for v in hist], "alpha": alpha}
 
 
+# ---------------------------------------------------------------- object filters
+def tv_denoise_amp(u: torch.Tensor, lam: float = 0.01, n_iter: int = 30, eps: float = 1e-8) -> torch.Tensor:
+    """Chambolle-type TV denoising of a real (ny, nx) float64 GPU array with periodic
+    differences, tau = 1/4 (ReconstructorPWE.tv_denoise_amp, pwe_reconstructor.py:370-386)."""
+    u = u.to(torch.float64).contiguous().clone()
+    ny, nx = u.shape
+    work = torch.empty(3 * ny * nx, dtype=torch.float64, device=u.device)
+    _lib.check(_lib.load().tp_tv_denoise(_lib.ptr(u), ny, nx, float(lam), int(n_iter), float(eps), _lib.ptr(work),
+                                         _lib.stream_handle()), "tp_tv_denoise")
+    return u
+
+
+def gaussian_lowpass_phase(phi: torch.Tensor, sigma: float = 0.08) -> torch.Tensor:
+    """real(ifft2(fft2(phi) * fftshift(W))) for a real (ny, nx) GPU array, W the
+    Gaussian of width sigma on linspace(-1/2, 1/2, n, endpoint=False) along each
+    axis (gaussian_lowpass_phase, pwe_reconstructor.py:388-400); any-length DFTs."""
+    from .forward_model import ms_engine
+
+    ny, nx = phi.shape
+    x = phi.to(torch.complex64).contiguous()
+    rows = ms_engine.dft_rows(x)
+    spec = torch.empty_like(rows)
+    ms_engine._dft(rows, nx, 1, spec, nx, 1, ny, nx)  # along axis 0
+    _lib.check(_lib.load().tp_lowpass_weight(_lib.ptr(spec), ny, nx, float(sigma), _lib.stream_handle()),
+               "tp_lowpass_weight")
+    back = torch.empty_like(spec)
+    ms_engine._dft(spec, nx, 1, back, nx, 1, ny, nx, inverse=True)
+    out = ms_engine.dft_rows(back, inverse=True)
+    return out.real.to(torch.float64) / (ny * nx)
+
+
+def global_object_update(n: torch.Tensor, lam_amp: float, sigma_phase: float) -> torch.Tensor:
+    """|n| TV-denoised and arg(n) low-passed, recombined (global_object_update,
+    pwe_reconstructor.py:402-419); n complex128 (ny, nx) on the GPU."""
+    n = n.to(torch.complex128).contiguous()
+    ny, nx = n.shape
+    A = torch.empty((ny, nx), dtype=torch.float64, device=n.device)
+    Phi = torch.empty_like(A)
+    lib = _lib.load()
+    _lib.check(lib.tp_object_polar(_lib.ptr(n), ny, nx, _lib.ptr(A), _lib.ptr(Phi), _lib.stream_handle()),
+               "tp_object_polar")
+    if lam_amp > 0.0:
+        A = tv_denoise_amp(A, lam=lam_amp)
+    if sigma_phase > 0.0:
+        Phi = gaussian_lowpass_phase(Phi, sigma=sigma_phase).contiguous()
+    out = torch.empty_like(n)
+    _lib.check(lib.tp_object_join(_lib.ptr(A), _lib.ptr(Phi), ny, nx, _lib.ptr(out), _lib.stream_handle()),
+               "tp_object_join")
+    return out
+
+
+# ---------------------------------------------------------------- the reference's NLCG loop
+class ReferenceNLCG:
+    """ReconstructorPWE.reconstruct (pwe_reconstructor.py:230-368) on the GPU, one
+    transverse axis, full domain: per iteration the forward field of every probe
+    (all planes), the exit-wave error (phase-retrieval constraint or exit - data,
+    base_reconstructor.py:87-121), the reference-form gradient
+    (compute_grad_least_squares with grad_Ak = -get_gradient(n): the reference
+    adjoint sweep with the gradient accumulation fused), the step of
+    compute_alphak (:166-219) or -fixed_step / (i + 1), beta = min(FR, max(PR, 0))
+    per part, and the optional TV / low-pass object update.
+
+    compute_alphak per part: perturb = grad_A u (grad_A = -g0 (I + subdiagonal
+    block), g0 = dz k / 2i, times i for the imaginary part), i.e. step j gets
+    -g0 (u_j + u_{j-1}) (u_1 alone at j = 1); one forward sweep of every probe
+    with that right-hand side; delta_p = -(exit plane) . d[-N:] without
+    conjugate; alpha = -<d, grad> / sum_p |delta_p|^2."""
+
+    def __init__(self, data, probes, k: float, dx: float, dz: float, bc: str = "dirichlet",
+                 phase_retrieval: bool = True, device=None, probes_per_block: int = 4):
+        from .forward_model import operators
+
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.probes = torch.as_tensor(np.asarray(probes)).to(self.dev, torch.complex64).contiguous()
+        P, N = self.probes.shape
+        self.phase_retrieval = phase_retrieval
+        d = np.asarray(data).reshape(P, N)
+        if phase_retrieval:
+            self.sqrt_i = torch.as_tensor(np.sqrt(d.real.astype(float))).to(self.dev, torch.float32).contiguous()
+        else:
+            self.exit_data = torch.as_tensor(d.astype(complex)).to(self.dev, torch.complex64)
+        self.k, self.dx, self.dz, self.bc, self.ppb = k, dx, dz, bc, probes_per_block
+        self.A, self.B = operators.step_operators_1d(N, k, dx, dz, bc)
+        self.g0 = dz * k / 2j
+
+    def _forward(self, fac) -> torch.Tensor:
+        from .forward_model import tri_engine
+
+        P, N = self.probes.shape
+        u = torch.empty((P, fac.steps + 1, N), dtype=torch.complex64, device=self.dev)
+        tri_engine.sweep([tri_engine.TriScanSpec(fac, self.probes, out=u)], probes_per_block=self.ppb)
+        return u
+
+    def _gradient(self, adj, u: torch.Tensor, err: torch.Tensor):
+        """(grad_re, grad_im) float64 (L*N,) of compute_grad_least_squares."""
+        from .forward_model import tri_engine
+
+        P, L1, N = u.shape
+        L = L1 - 1
+        G = torch.zeros((L, N), dtype=torch.complex64, device=self.dev)
+        lam = torch.zeros((P, N), dtype=torch.complex64, device=self.dev)
+        spec = tri_engine.TriScanSpec(adj, lam, exit=lam, rhs=err.reshape(P, 1, N).contiguous(), rhs_steps=1,
+                                      gfwd=u, gfwd_z0=L, gfwd_zdir=-1, grad=G, grad_z0=L - 1, grad_zdir=-1,
+                                      grad_skip_row0=True)
+        tri_engine.sweep([spec], probes_per_block=self.ppb if self.ppb in (1, 4) else 4)
+        return tg.reference_gradient_parts(G, self.k, self.dz)
+
+    def _alphak(self, fwd, u: torch.Tensor, scale: complex, grad_E: torch.Tensor, d: torch.Tensor) -> float:
+        from .forward_model import tri_engine
+
+        P, L1, N = u.shape
+        pert = u[:, 1:] + u[:, :-1]
+        pert[:, 0] = u[:, 1]
+        pert *= complex(-self.g0 * scale)
+        ex = torch.empty((P, N), dtype=torch.complex64, device=self.dev)
+        tri_engine.sweep([tri_engine.TriScanSpec(fwd, self.probes, exit=ex, rhs=pert)], probes_per_block=self.ppb)
+        dp = -(ex.to(torch.complex128) * d[-N:].to(torch.complex128)).sum(dim=1)
+        den = float((dp.abs() ** 2).sum())
+        num = float(torch.dot(d, grad_E))
+        return -(num / den) if den > 0 else 0.0
+
+    def run(self, n0, max_iters: int = 10, tol: float = 1e-8, fixed_step_size=None, tv_lambda_amp: float = 0.0,
+            low_pass_sigma_phase: float = 0.0, log=None) -> dict:
+        """{'n_est': complex128 (N, nz), 'wave_field': the last forward field,
+        complex64 (P, N, nz-1) (planes 1..L), 'residual_history': exit-wave RMSE per
+        iteration, 'steps': [(alpha_re, alpha_im)]}."""
+        from .forward_model import tri_engine
+
+        nk = torch.as_tensor(np.asarray(n0)).to(self.dev, torch.complex128).clone()
+        P, N = self.probes.shape
+        L = nk.shape[1] - 1
+        residual, steps = [], []
+        dk_re = dk_im = g_re_old = g_im_old = None
+        u = None
+        for i in range(max_iters):
+            C = tg.object_contribution(nk, self.k, self.dz)
+            fwd = tri_engine.factor(C, self.A, self.B, "forward")
+            adj = tri_engine.factor(C, self.A, self.B, "adjoint")
+            u = self._forward(fwd)
+            ex = u[:, L].contiguous()
+            if self.phase_retrieval:
+                err, _ = tg.amplitude_seed(ex, self.sqrt_i, 1.0 / N)
+            else:
+                err = ex - self.exit_data
+            residual.append(float(torch.sqrt((err.abs().double() ** 2).mean())))
+            if residual[i] < tol:
+                break
+            g_re, g_im = self._gradient(adj, u, err)
+            if i == 0:
+                dk_re, dk_im = -g_re, -g_im
+                g_re_old, g_im_old = g_re, g_im
+            if fixed_step_size is not None:
+                a_re = a_im = -fixed_step_size / (i + 1)
+            else:
+                a_re = self._alphak(fwd, u, 1.0, g_re, dk_re)
+                a_im = self._alphak(fwd, u, 1j, g_im, dk_im)
+            steps.append((a_re, a_im))
+            upd = torch.complex(a_re * dk_re, a_im * dk_im).reshape(L, N).transpose(0, 1)
+            nk[:, 1:] += upd
+            b_re, b_im = beta_rule(g_re, g_re_old), beta_rule(g_im, g_im_old)
+            dk_re = -g_re + b_re * dk_re
+            g_re_old = g_re
+            dk_im = -g_im + b_im * dk_im
+            g_im_old = g_im
+            if tv_lambda_amp > 0.0 or low_pass_sigma_phase > 0.0:
+                nk = global_object_update(nk, tv_lambda_amp, low_pass_sigma_phase)
+            if log:
+                log(f"iteration {i + 1}: rmse {residual[i]:.6e} alpha ({a_re:.4e}, {a_im:.4e})")
+        wf = u[:, 1:].transpose(1, 2) if u is not None else None
+        return {"n_est": nk, "wave_field": wf, "residual_history": residual, "steps": steps}
+
+
 class ReconstructorPWE:
     """The reference's entry point for one transverse axis (ReconstructorPWE,
-    pwe_reconstructor.py:34-368) on ``Reconstruction2D``: ``reconstruct`` returns
-    (n_est, wave_field, residual) with the reference's residual (exit-wave RMSE)
-    and stops once it is below ``tol``.  ``probes`` are given explicitly (the
+    pwe_reconstructor.py:34-368): ``reconstruct`` returns (n_est, wave_field,
+    residual) with the reference's residual (exit-wave RMSE per iteration) and
+    stops once it is below ``tol``.  ``probes`` are given explicitly (the
     reference builds them from its simulation config); ``data`` are intensities,
-    or exit waves with ``phase_retrieval=False``; the step is 0.5 / Lipschitz, or
-    ``fixed_step_size`` as a constant step; the TV / low-pass object filters are
-    not built."""
+    or exit waves with ``phase_retrieval=False``.
+
+    method="reference" (full domain, one projection): the reference's loop on
+    the GPU (``ReferenceNLCG``: its gradient, compute_alphak or
+    -fixed_step_size / (i + 1), beta rule and the TV / low-pass object update),
+    wave_field (1, 1, P, N, nz-1).  method="lipschitz" (and every reduced-domain
+    space): NLCG on the exact gradient with step 0.5 / Lipschitz (or
+    ``fixed_step_size``), ``Reconstruction2D``; the object filters apply after
+    each iteration there too."""
 
     def __init__(self, simulation_space, data, probes, phase_retrieval: bool = True, solver_type: str = "iterative",
                  bc_type="impedance", verbose: Optional[bool] = None, device=None):
@@ -179,16 +362,33 @@ class ReconstructorPWE:
         self.ss = ss
         windows = list(getattr(ss, "_scan_windows", None) or []) if ss.solve_reduced_domain else None
         W = ss.effective_nx if windows else ss.nx
-        self.rc = Reconstruction2D(np.asarray(data).reshape(-1, W), np.asarray(probes).reshape(-1, W), ss.k,
-                                   ss.dx, ss.dz, bc_type, device=device, phase_retrieval=phase_retrieval,
-                                   windows=windows)
+        if getattr(ss, "exact_ref_coeff", False):
+            raise NotImplementedError("exact_ref_coeff objects: the GPU reconstruction uses the default object "
+                                      "coefficient (k/i)(n - 1)")
+        self.bc_type = bc_type
+        self._data = np.asarray(data).reshape(-1, W)
+        self._probes = np.asarray(probes).reshape(-1, W)
+        self.rc = Reconstruction2D(self._data, self._probes, ss.k, ss.dx, ss.dz, bc_type, device=device,
+                                   phase_retrieval=phase_retrieval, windows=windows)
 
     def reconstruct(self, max_iters: int = 10, tol: float = 1e-8, n_initial=None, fixed_step_size=None,
-                    tv_lambda_amp: float = 0.0, low_pass_sigma_phase: float = 0.0):
-        if tv_lambda_amp or low_pass_sigma_phase:
-            raise NotImplementedError("TV / low-pass object filters are not built")
+                    tv_lambda_amp: float = 0.0, low_pass_sigma_phase: float = 0.0, method: str = "reference"):
+        if method not in ("reference", "lipschitz"):
+            raise ValueError("method must be 'reference' or 'lipschitz'")
         n0 = n_initial if n_initial is not None else np.ones(self.ss.shape, dtype=complex) * self.ss.n_medium
-        out = self.rc.run(n0, max_iters, method="nlcg", alpha=fixed_step_size, tol=tol)
+        if method == "reference" and self.rc.windows is None:
+            rc = ReferenceNLCG(self._data, self._probes, self.ss.k, self.ss.dx, self.ss.dz, self.bc_type,
+                               phase_retrieval=self.rc.phase_retrieval, device=self.rc.dev)
+            out = rc.run(n0, max_iters, tol=tol, fixed_step_size=fixed_step_size, tv_lambda_amp=tv_lambda_amp,
+                         low_pass_sigma_phase=low_pass_sigma_phase)
+            wf = out["wave_field"]
+            wf = None if wf is None else wf[None, None].cpu().numpy()
+            return out["n_est"].cpu().numpy(), wf, out["residual_history"]
+        post = None
+        if tv_lambda_amp > 0.0 or low_pass_sigma_phase > 0.0:
+            def post(n):
+                return global_object_update(n, tv_lambda_amp, low_pass_sigma_phase)
+        out = self.rc.run(n0, max_iters, method="nlcg", alpha=fixed_step_size, tol=tol, post=post)
         return out["n_est"], out["wave_field"], out["rmse_history"]
 
 
