@@ -330,6 +330,37 @@ int tp_dft_any(const void* d_in, int64_t in_stride, int64_t in_dist, void* d_out
 int tp_twiddle_mul(void* d_x, int A, int B, int count, int64_t dist, int sign, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Refinement of the forward sweep (residual and seed beyond complex64)
+ *
+ * The exit plane of the amplitude-misfit gradient (pwe_reconstructor.py:100-164,
+ * base_reconstructor.py:87-207) is needed beyond complex64: the loss amplifies
+ * exit-plane errors in far-field bins many orders of magnitude below the peak.
+ * A refinement step is the residual of the FP64 recurrence at the current
+ * iterate, then a complex64 sweep of the same factors on it (tp_tri_sweep with
+ * rhs); the iterate is the sum of the complex64 planes of the sweep and its
+ * corrections.  These two calls are the library's FP64 arithmetic besides the
+ * factor pivots and the Krylov scalars.
+ * ------------------------------------------------------------------------ */
+
+/* r_s = 2 v_{s-1} - A_s (v_{s-1} + v_s) in FP64 for local steps s = 1..steps
+ * (sigma = 2 modes, B_s = 2I - A_s), v = u + u1 + u2 (complex64 plane sets with
+ * the same strides; u1, u2 may be NULL), A(s) as in tp_tri_factor at global step
+ * s_first + s: tridiag(lhs_sub, lhs_diag + lhs_sign * c_l, lhs_sup) from the
+ * FP64 d_C and d_base.  Plane t at u + p*u_pstride + t*u_zstride (t = 0..steps);
+ * r: complex64, step s at r + p*r_pstride + (s-1)*r_sstride. */
+int tp_tri_residual(const void* d_u, const void* d_u1, const void* d_u2, int64_t u_pstride,
+                    int64_t u_zstride, int probes, int n, int steps, const void* d_C, int ncol,
+                    const void* d_base, double lhs_sign, int conj_c, int lhs_col0, int col_step,
+                    int s_first, void* d_r, int64_t r_pstride, int64_t r_sstride, void* stream);
+
+/* tp_amplitude_seed for an exit plane x = x0 + x1 + x2 (complex64 each, row
+ * stride ld; x1, x2 may be NULL) and amplitudes sqrt_i + sqrt_i_lo (float32;
+ * sqrt_i_lo may be NULL), both FFTs in FP64; n a power of two from 64 to 8192. */
+int tp_amplitude_seed_refined(const void* d_x0, const void* d_x1, const void* d_x2, int64_t ld,
+                              const void* d_sqrt_i, const void* d_sqrt_i_lo, int n, int probes,
+                              double scale, void* d_seed, void* d_loss, void* stream);
+
+/* ------------------------------------------------------------------------
  * Object filters of the one-axis reconstruction loop
  * (ReconstructorPWE.tv_denoise_amp / gaussian_lowpass_phase /
  * global_object_update, thick_ptycho/reconstruction/pwe_reconstructor.py:370-419).

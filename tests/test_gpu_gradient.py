@@ -113,46 +113,90 @@ def test_autograd_backward(gpu):
     assert rel_l2(n.grad.cpu().numpy(), ref) < 1e-4
 
 
-def test_config2_gradient(gpu):
-    """Config 2: Nx = 1024, Nz = 512, 256 probes, checkpoints every 32 slices; measured data
-    from the true object with 5 % noise on delta-n, gradient at the seeded initial guess.
-
-    The amplitude misfit at this point amplifies exit-wave perturbations about 1e4-fold:
-    the FP64 gradient computed from the FP64 exit waves rounded to complex64 is already
-    1.5e-3 (rel-L2) from the FP64 one, and from the GPU's complex64 exit waves (rel. 2e-5)
-    4e-3.  Checked here:
-      * the adjoint stage, fed the FP64 loss seed: rel-L2 <= 2e-5;
-      * the full GPU pipeline against the FP64 gradient of the GPU's own exit waves
-        (rel-L2 <= 1e-4), and against the FP64 oracle no worse than that gradient is
-        (within 1e-4)."""
+def _config2():
     wl = workloads.config2()
     sc = wl.scans[0]
     g = pwe.Grid(dimension=2, nx=sc.nx, nz=sc.nz, dx=sc.dx, dz=sc.dz, k=sc.k)
     fwd = pwe.build_lu_cache(g, sc.bc, wl.extra["n_meas_object"], "forward")
     I = np.abs(np.fft.fft(pwe.solve_batched(g, fwd, sc.probes, keep="exit"), axis=-1)) ** 2
+    return sc, g, I
+
+
+def test_config2_gradient(gpu):
+    """Config 2: Nx = 1024, Nz = 512, 256 probes, checkpoints every 32 slices; measured data
+    from the true object with 5 % noise on delta-n, gradient at the seeded initial guess,
+    against the FP64 oracle (rel-L2 <= 1e-4).
+
+    The amplitude loss at this point is set by far-field bins down to 1e-17 of the peak
+    where the data are ~3e-5 of it: perturbing the FP64 exit plane by 1e-15 (2e-10) of its
+    peak moves the FP64 gradient by 2.5e-5 (7e-4).  The pipeline refines the exit plane by
+    two steps (FP64 residuals) and forms the seed in FP64; the complex64 plane alone gives
+    ~4e-3.  Also checked: the adjoint stage fed the FP64 seed (<= 2e-5) and the loss."""
+    sc, g, I = _config2()
     ref, J = og.exact_gradient(g, sc.bc, sc.n, sc.probes, I)
     ex = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), sc.probes, keep="exit")
 
-    def run(seed=None):
+    def run(seed=None, refine=True):
         loss, grad = tg.amplitude_gradient(sc.n, sc.probes, np.sqrt(I), sc.k, sc.dx, sc.dz, sc.bc, checkpoint=32,
-                                           seed=seed)
+                                           seed=seed, refine=refine)
         return loss, grad.cpu().numpy()
 
     loss, grad = run()
-    assert abs(loss - J) <= 1e-4 * J
+    assert abs(loss - J) <= 1e-5 * J
     _, grad_adj = run(og.amplitude_loss_seed(ex, I))
-    # FP64 gradient at the GPU's exit waves: the FP64 adjoint states of the FP64 seed of those waves
+    _, grad_plain = run(refine=False)
+    e_adj, e_full, e_plain = rel_l2(grad_adj, ref), rel_l2(grad, ref), rel_l2(grad_plain, ref)
+    print(f"config 2 gradient rel-L2: refined {e_full:.2e}, adjoint stage {e_adj:.2e}, unrefined {e_plain:.2e}")
+    assert e_adj < 2e-5
+    assert e_full < 1e-4
+
+
+def test_config2_gradient_central_differences(gpu):
+    """Config 2 gradient against FP64 central differences of the oracle loss
+    (h = 1e-6) at 6 seeded entries of n (3 real, 3 imaginary parts)."""
+    sc, g, I = _config2()
+    _, grad = tg.amplitude_gradient(sc.n, sc.probes, np.sqrt(I), sc.k, sc.dx, sc.dz, sc.bc, checkpoint=32)
+    grad = grad.cpu().numpy()
+    scale = np.max(np.abs(grad))
+    rng = np.random.default_rng(64)
+    h = 1e-6
+
+    def loss(n):
+        u = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, n, "forward"), sc.probes, keep="exit")
+        return og.amplitude_loss(u, I)
+
+    # entries where the gradient is within two decades of its peak (the probes' footprint)
+    cand = np.argwhere(np.abs(grad) > 1e-2 * scale)
+    for i, (ix, iz) in enumerate(cand[rng.choice(len(cand), 6, replace=False)]):
+        d = 1.0 if i % 2 == 0 else 1j
+        npl, nmi = sc.n.copy(), sc.n.copy()
+        npl[ix, iz] += h * d
+        nmi[ix, iz] -= h * d
+        fd = (loss(npl) - loss(nmi)) / (2 * h)
+        an = grad[ix, iz].real if i % 2 == 0 else grad[ix, iz].imag
+        print(f"entry ({ix}, {iz}) {'re' if i % 2 == 0 else 'im'}: fd {fd:.6e} gpu {an:.6e}")
+        assert abs(fd - an) <= 1e-4 * scale + 1e-3 * abs(fd), (ix, iz, fd, an)
+
+
+def test_refined_exit_plane(gpu):
+    """Refinement steps take the complex64 exit plane (rel. 2e-5 from FP64 after
+    511 slices) to ~2e-10 (one step) and to the FP64 level (two), with the probes'
+    own rounding carried in."""
+    sc, g, _ = _config2()
     dev = torch.device("cuda")
+    ref = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), sc.probes, keep="exit")
     A, B = tg._operators(sc.nx, sc.k, sc.dx, sc.dz, sc.bc)
     C = tg.object_contribution(torch.tensor(sc.n, dtype=torch.complex128, device=dev), sc.k, sc.dz)
-    _, ex_gpu = tg._Sweeper(C, A, B, "exact_adjoint", 32).forward(torch.tensor(sc.probes, dtype=torch.complex64,
-                                                                              device=dev))
-    _, grad_gpu_exit = run(og.amplitude_loss_seed(ex_gpu.cpu().numpy().astype(complex), I))
-    e_adj, e_full, e_floor = rel_l2(grad_adj, ref), rel_l2(grad, ref), rel_l2(grad_gpu_exit, ref)
-    print(f"config 2 gradient rel-L2: adjoint stage {e_adj:.2e}, full {e_full:.2e}, FP64 at GPU exit {e_floor:.2e}")
-    assert e_adj < 2e-5
-    assert rel_l2(grad, grad_gpu_exit) < 1e-4
-    assert e_full <= e_floor + 1e-4
+    sw = tg._Sweeper(C, A, B, "exact_adjoint", 32)
+    hi, lo = tg._probe_parts(sc.probes, dev, sc.nx)
+    ck, ex = sw.forward(hi)
+    plain = rel_l2(ex.cpu().numpy(), ref)
+    e1 = sw.refine_exit(ck, lo, steps=1)
+    one = rel_l2(ex.cpu().numpy().astype(complex) + e1[0].cpu().numpy(), ref)
+    e12 = sw.refine_exit(ck, lo, steps=2)
+    two = rel_l2(ex.cpu().numpy().astype(complex) + e12[0].cpu().numpy() + e12[1].cpu().numpy(), ref)
+    print(f"exit plane rel-L2: complex64 {plain:.2e}, one step {one:.2e}, two steps {two:.2e}")
+    assert one < 1e-9 and two < 1e-13
 
 
 def test_amplitude_seed_kernel(gpu):

@@ -174,6 +174,11 @@ SIGNATURES = {
                            c_void_p, c_void_p, c_void_p, c_void_p]),
     "tp_twiddle_mul": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int, c_void_p]),
     "tp_object_polar": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "tp_tri_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int, c_int, c_int, c_void_p,
+                                c_int, c_void_p, c_double, c_int, c_int, c_int, c_int, c_void_p, c_int64, c_int64,
+                                c_void_p]),
+    "tp_amplitude_seed_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int,
+                                          c_double, c_void_p, c_void_p, c_void_p]),
     "tp_object_join": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tp_tv_denoise": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, c_void_p, c_void_p]),
     "tp_lowpass_weight": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p]),

@@ -37,6 +37,8 @@ class TriFactors:
     keep: list = field(default_factory=list)
     waves: int = 1  # wavefronts per probe (wide layout only)
     wide: bool = False  # factored for tp_tri_sweep_wide (row i at (i % chunk) * 64 * waves + i / chunk)
+    plan: Optional[ModePlan] = None  # the operator it was factored from (for residuals)
+    first: int = 0  # steps of the full factor set before this one (step_slice views)
 
     def check(self) -> None:
         """Raise if a pivot was exactly zero (synchronises)."""
@@ -109,7 +111,8 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
                               _lib.ptr(status), sh),
             "tp_tri_factor",
         )
-    return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C], waves=nw, wide=wide)
+    return TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C], waves=nw, wide=wide,
+                      plan=plan)
 
 
 def coef_elems(L: int, chunk: int, waves: int = 1) -> int:
@@ -243,7 +246,37 @@ def step_slice(fac: TriFactors, first: int, count: int) -> TriFactors:
     assert 0 <= first and first + count <= fac.steps and count >= 1
     per = coef_elems(1, fac.chunk, fac.waves)
     return TriFactors(fac.coef[first * per:(first + count) * per], fac.n, count, fac.chunk, fac.mode, fac.b_sign,
-                      fac.status, fac.sigma, keep=[fac], waves=fac.waves, wide=fac.wide)
+                      fac.status, fac.sigma, keep=[fac], waves=fac.waves, wide=fac.wide, plan=fac.plan,
+                      first=fac.first + first)
+
+
+def residual(fac: TriFactors, planes, stream=None) -> torch.Tensor:
+    """r_s = 2 v_{s-1} - A_s (v_{s-1} + v_s) in FP64 (tp_tri_residual) for the
+    steps of ``fac`` (a full factor set or a step_slice view): the residual of the
+    FP64 recurrence at the iterate v, given as one complex64 (P, steps+1, n)
+    plane set or a list of up to three with equal strides whose sum is v (plane
+    0 the state before the view's first step).  Returns complex64 (P, steps, n),
+    the right-hand side of a refinement sweep of the same factors."""
+    if fac.plan is None or fac.sigma:
+        raise ValueError("residuals are built for factors of the sigma = 2 modes from tri_engine.factor")
+    parts = list(planes) if isinstance(planes, (list, tuple)) else [planes]
+    assert 1 <= len(parts) <= 3
+    P, S1, n = parts[0].shape
+    for q in parts:
+        assert q.shape == parts[0].shape and q.stride() == parts[0].stride() and q.dtype == torch.complex64
+    assert S1 == fac.steps + 1 and n == fac.n and parts[0].stride(-1) == 1
+    root = fac
+    while root.keep and isinstance(root.keep[0], TriFactors):  # step_slice views keep their parent
+        root = root.keep[0]
+    base, C = root.keep[:2]
+    plan = fac.plan
+    ptrs = [_lib.ptr(q) for q in parts] + [None] * (3 - len(parts))
+    r = torch.empty((P, fac.steps, n), dtype=torch.complex64, device=parts[0].device)
+    _lib.check(_lib.load().tp_tri_residual(ptrs[0], ptrs[1], ptrs[2], parts[0].stride(0), parts[0].stride(1), P, n,
+                                           fac.steps, _lib.ptr(C), C.shape[1], _lib.ptr(base), plan.lhs_sign,
+                                           plan.conj_c, plan.lhs_col0, plan.col_step, fac.first, _lib.ptr(r),
+                                           r.stride(0), r.stride(1), _lib.stream_handle(stream)), "tp_tri_residual")
+    return r
 
 
 def _structs(specs: Sequence[TriScanSpec], allow_half_chunk: bool = False):

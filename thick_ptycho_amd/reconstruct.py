@@ -237,12 +237,13 @@ class ReferenceNLCG:
         from .forward_model import operators
 
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
-        self.probes = torch.as_tensor(np.asarray(probes)).to(self.dev, torch.complex64).contiguous()
+        pr = np.asarray(probes)
+        self.probes, self.probes_lo = tg._probe_parts(pr, self.dev, pr.shape[-1])
         P, N = self.probes.shape
         self.phase_retrieval = phase_retrieval
         d = np.asarray(data).reshape(P, N)
         if phase_retrieval:
-            self.sqrt_i = torch.as_tensor(np.sqrt(d.real.astype(float))).to(self.dev, torch.float32).contiguous()
+            self.sqrt_i, self.sqrt_i_lo = tg.amplitude_parts(np.sqrt(d.real.astype(float)), self.dev, (P, N))
         else:
             self.exit_data = torch.as_tensor(d.astype(complex)).to(self.dev, torch.complex64)
         self.k, self.dx, self.dz, self.bc, self.ppb = k, dx, dz, bc, probes_per_block
@@ -285,11 +286,29 @@ class ReferenceNLCG:
         num = float(torch.dot(d, grad_E))
         return -(num / den) if den > 0 else 0.0
 
+    def _refined_exit(self, fwd, u: torch.Tensor) -> list:
+        """Corrections of the exit plane by refinement steps over the full field
+        (tri_engine.residual at the iterate, then a sweep of the same factors:
+        first from the probes' rounding error, then from zero)."""
+        from .forward_model import tri_engine
+
+        e1 = torch.empty_like(u)
+        x1 = torch.empty_like(self.probes)
+        tri_engine.sweep([tri_engine.TriScanSpec(fwd, self.probes_lo, rhs=tri_engine.residual(fwd, u), out=e1,
+                                                 exit=x1)], probes_per_block=self.ppb)
+        x2 = torch.empty_like(self.probes)
+        zero = torch.zeros_like(self.probes)
+        tri_engine.sweep([tri_engine.TriScanSpec(fwd, zero, rhs=tri_engine.residual(fwd, [u, e1]), exit=x2)],
+                         probes_per_block=self.ppb)
+        return [x1, x2]
+
     def run(self, n0, max_iters: int = 10, tol: float = 1e-8, fixed_step_size=None, tv_lambda_amp: float = 0.0,
-            low_pass_sigma_phase: float = 0.0, log=None) -> dict:
+            low_pass_sigma_phase: float = 0.0, log=None, refine: bool = True) -> dict:
         """{'n_est': complex128 (N, nz), 'wave_field': the last forward field,
         complex64 (P, N, nz-1) (planes 1..L), 'residual_history': exit-wave RMSE per
-        iteration, 'steps': [(alpha_re, alpha_im)]}."""
+        iteration, 'steps': [(alpha_re, alpha_im)]}.  ``refine`` forms the
+        exit-wave error from the exit plane after two refinement steps (FP64
+        seed for the phase-retrieval error)."""
         from .forward_model import tri_engine
 
         nk = torch.as_tensor(np.asarray(n0)).to(self.dev, torch.complex128).clone()
@@ -304,10 +323,11 @@ class ReferenceNLCG:
             adj = tri_engine.factor(C, self.A, self.B, "adjoint")
             u = self._forward(fwd)
             ex = u[:, L].contiguous()
+            ex_lo = self._refined_exit(fwd, u) if refine else None
             if self.phase_retrieval:
-                err, _ = tg.amplitude_seed(ex, self.sqrt_i, 1.0 / N)
+                err, _ = tg.amplitude_seed(ex, self.sqrt_i, 1.0 / N, exit_lo=ex_lo, sqrt_i_lo=self.sqrt_i_lo)
             else:
-                err = ex - self.exit_data
+                err = ex - self.exit_data if ex_lo is None else (ex - self.exit_data) + sum(ex_lo)
             residual.append(float(torch.sqrt((err.abs().double() ** 2).mean())))
             if residual[i] < tol:
                 break
