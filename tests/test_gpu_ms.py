@@ -119,5 +119,7 @@ def test_farfield_2d_matches_numpy(gpu, rng):
         got = ms_engine.farfield_2d(torch.tensor(ex, dtype=torch.complex64, device="cuda"), nx, ny).cpu().numpy()
         ref = oms.farfield_per_probe_2d(ex.reshape(4, -1), nx, ny).reshape(4, nx, ny)
         assert rel_l2(got, ref) < TOL
-    with pytest.raises(ValueError):
-        ms_engine.farfield_2d(torch.zeros((1, 96 * 64), dtype=torch.complex64, device="cuda"), 96, 64)
+    # planes that are not powers of two take the any-length DFT along each axis
+    ex = rng.standard_normal((2, 96, 64)) + 1j * rng.standard_normal((2, 96, 64))
+    got = ms_engine.farfield_2d(torch.tensor(ex, dtype=torch.complex64, device="cuda"), 96, 64).cpu().numpy()
+    assert rel_l2(got, np.abs(np.fft.fft2(ex)) ** 2) < TOL
