@@ -378,6 +378,18 @@ int tp_object_join(const void* d_A, const void* d_Phi, int ny, int nx, void* d_n
  * d_work: 3 * ny * nx float64. */
 int tp_tv_denoise(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* d_work, void* stream);
 
+/* tp_tv_denoise without a caller work buffer and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The work space
+ * is a file-scope device array of 3 * TP_TV_NOALLOC_MAX_POINTS float64 (48 MiB),
+ * one per GPU the library runs on, held until the process ends; planes of more
+ * than TP_TV_NOALLOC_MAX_POINTS points return TP_ERR_INVALID.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other host
+ * threads, other streams) race and must be serialised by the caller. */
+#define TP_TV_NOALLOC_MAX_POINTS (1 << 21)
+#define TP_NOALLOC_MAX_DEVICES 64
+int tp_tv_denoise_noalloc(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* stream);
+
 /* In place on a complex64 spectrum X (ny, nx) in FFT order: X *= fftshift(W),
  * W = exp(-(qx^2 + qy^2) / (2 sigma^2)), q = linspace(-1/2, 1/2, n, endpoint=False). */
 int tp_lowpass_weight(void* d_X, int ny, int nx, double sigma, void* stream);

@@ -181,6 +181,7 @@ SIGNATURES = {
                                           c_double, c_void_p, c_void_p, c_void_p]),
     "tp_object_join": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tp_tv_denoise": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, c_void_p, c_void_p]),
+    "tp_tv_denoise_noalloc": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, c_void_p]),
     "tp_lowpass_weight": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p]),
     "tp_band_coef_bytes": (c_int64, [c_int, c_int, c_int, c_int]),
     "tp_band_factor": (

@@ -9,6 +9,8 @@
 //   tp_object_join:    n = A exp(i Phi)
 // Reference: ReconstructorPWE.tv_denoise_amp / gaussian_lowpass_phase /
 // global_object_update (thick_ptycho/reconstruction/pwe_reconstructor.py:370-419).
+#include <mutex>
+
 #include "tp_common.h"
 
 namespace tp {
@@ -107,18 +109,13 @@ extern "C" int tp_object_join(const void* d_A, const void* d_Phi, int ny, int nx
   return TP_OK;
 }
 
-extern "C" int tp_tv_denoise(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* d_work,
-                             void* stream) {
-  if (!d_u || !d_work || ny < 1 || nx < 1 || n_iter < 0 || !(lam > 0.0)) {
-    set_error("tp_tv_denoise: invalid arguments (ny=%d, nx=%d, lam=%g, n_iter=%d)", ny, nx, lam, n_iter);
-    return TP_ERR_INVALID;
-  }
+// The TV iterations on u with the dual field and the work plane in `work`
+// (3 * ny * nx float64); shared by tp_tv_denoise and tp_tv_denoise_noalloc.
+static int tv_run(double* u, int ny, int nx, double lam, int n_iter, double eps, double* work, hipStream_t st) {
   const int64_t total = (int64_t)ny * nx;
-  double* u = (double*)d_u;
-  double* px = (double*)d_work;
+  double* px = work;
   double* py = px + total;
   double* uxy = py + total;
-  hipStream_t st = (hipStream_t)stream;
   const dim3 g = grid_for(total);
   // fill kernels rather than hipMemsetAsync / hipMemcpyAsync: the calls stay capturable into HIP graphs
   hipLaunchKernelGGL(fill_zero_kernel, grid_for(2 * total), dim3(256), 0, st, px, 2 * total);
@@ -132,6 +129,63 @@ extern "C" int tp_tv_denoise(void* d_u, int ny, int nx, double lam, int n_iter, 
   hipLaunchKernelGGL(copy_kernel, g, dim3(256), 0, st, (const double*)uxy, u, total);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
+}
+
+extern "C" int tp_tv_denoise(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* d_work,
+                             void* stream) {
+  if (!d_u || !d_work || ny < 1 || nx < 1 || n_iter < 0 || !(lam > 0.0)) {
+    set_error("tp_tv_denoise: invalid arguments (ny=%d, nx=%d, lam=%g, n_iter=%d)", ny, nx, lam, n_iter);
+    return TP_ERR_INVALID;
+  }
+  return tv_run((double*)d_u, ny, nx, lam, n_iter, eps, (double*)d_work, (hipStream_t)stream);
+}
+
+// Work space of tp_tv_denoise_noalloc: one copy per GPU, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host it is
+// only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[3 * (size_t)TP_TV_NOALLOC_MAX_POINTS];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[TP_NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int tv_scratch(double** out) {
+  int dev = -1;
+  TP_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= TP_NOALLOC_MAX_DEVICES) {
+    set_error("tp_tv_denoise_noalloc: device %d beyond the %d this library tracks", dev, TP_NOALLOC_MAX_DEVICES);
+    return TP_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    TP_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("tp_tv_denoise_noalloc: no device address for the scratch array on device %d", dev);
+      return TP_ERR_HIP;
+    }
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return TP_OK;
+}
+
+extern "C" int tp_tv_denoise_noalloc(void* d_u, int ny, int nx, double lam, int n_iter, double eps, void* stream) {
+  if (!d_u || ny < 1 || nx < 1 || n_iter < 0 || !(lam > 0.0)) {
+    set_error("tp_tv_denoise_noalloc: invalid arguments (ny=%d, nx=%d, lam=%g, n_iter=%d)", ny, nx, lam, n_iter);
+    return TP_ERR_INVALID;
+  }
+  if ((int64_t)ny * nx > (int64_t)TP_TV_NOALLOC_MAX_POINTS) {
+    set_error("tp_tv_denoise_noalloc: %d x %d = %lld points exceed the built-in work space of %d points; "
+              "use tp_tv_denoise with a caller work buffer",
+              ny, nx, (long long)ny * nx, TP_TV_NOALLOC_MAX_POINTS);
+    return TP_ERR_INVALID;
+  }
+  double* work = nullptr;
+  const int rc = tv_scratch(&work);
+  if (rc != TP_OK) return rc;
+  return tv_run((double*)d_u, ny, nx, lam, n_iter, eps, work, (hipStream_t)stream);
 }
 
 extern "C" int tp_lowpass_weight(void* d_X, int ny, int nx, double sigma, void* stream) {
