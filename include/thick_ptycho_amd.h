@@ -360,6 +360,27 @@ int tp_amplitude_seed_refined(const void* d_x0, const void* d_x1, const void* d_
                               const void* d_sqrt_i, const void* d_sqrt_i_lo, int n, int probes,
                               double scale, void* d_seed, void* d_loss, void* stream);
 
+/* tp_tri_residual for two transverse axes (the block-tridiagonal A_s of
+ * tp_band_factor, operator_matrices.py:75-126): d_tri complex128 [3][3][nx]
+ * (block type first / interior / last; sub, diag, sup), d_cpl complex128
+ * [ny][2] (f_j, e_j), C element (i, col) at d_C[i * c_istride + col * c_cstride]
+ * (complex128).  Planes of nx * ny values; probes, steps <= 65535. */
+int tp_band_residual(const void* d_u, const void* d_u1, const void* d_u2, int64_t u_pstride,
+                     int64_t u_zstride, int probes, int nx, int ny, int steps, const void* d_C,
+                     int64_t c_istride, int64_t c_cstride, int ncol, const void* d_tri, const void* d_cpl,
+                     double lhs_sign, int conj_c, int lhs_col0, int col_step, int s_first, void* d_r,
+                     int64_t r_pstride, int64_t r_sstride, void* stream);
+
+/* tp_amplitude_seed2d (nx rows of ny values per probe, fft2 over both axes) for
+ * an exit plane x0 + x1 + x2 (complex64, probe stride ld; x1, x2 may be NULL)
+ * and amplitudes sqrt_i + sqrt_i_lo (float32; sqrt_i_lo may be NULL), both
+ * transforms in FP64; nx, ny powers of two from 8 to 2048.  d_work:
+ * tp_amplitude_seed2d_refined_work_bytes(nx, ny, probes). */
+size_t tp_amplitude_seed2d_refined_work_bytes(int nx, int ny, int probes);
+int tp_amplitude_seed2d_refined(const void* d_x0, const void* d_x1, const void* d_x2, int64_t ld,
+                                const void* d_sqrt_i, const void* d_sqrt_i_lo, int nx, int ny, int probes,
+                                double scale, void* d_seed, void* d_loss, void* d_work, void* stream);
+
 /* ------------------------------------------------------------------------
  * Object filters of the one-axis reconstruction loop
  * (ReconstructorPWE.tv_denoise_amp / gaussian_lowpass_phase /

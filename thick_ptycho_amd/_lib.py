@@ -179,6 +179,12 @@ SIGNATURES = {
                                 c_void_p]),
     "tp_amplitude_seed_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int,
                                           c_double, c_void_p, c_void_p, c_void_p]),
+    "tp_band_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int, c_int, c_int, c_int,
+                                 c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
+                                 c_int, c_void_p, c_int64, c_int64, c_void_p]),
+    "tp_amplitude_seed2d_refined_work_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "tp_amplitude_seed2d_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int,
+                                            c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "tp_object_join": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tp_tv_denoise": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, c_void_p, c_void_p]),
     "tp_tv_denoise_noalloc": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, c_void_p]),

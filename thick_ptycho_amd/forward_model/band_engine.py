@@ -317,6 +317,7 @@ class BandScanSpec:
     rhs: Optional[torch.Tensor] = None
     rhs_steps: int = 0
     flip: bool = False
+    work: Optional[torch.Tensor] = None  # (P, 2, N) ping-pong planes when out is None (allocated if not given)
     _keep: list = field(default_factory=list)
 
     def struct(self) -> _lib.BandScan:
@@ -338,8 +339,11 @@ class BandScanSpec:
             s.out_z0 = f.steps if self.flip else 0
             s.out_zdir = -1 if self.flip else 1
         else:
-            work = torch.empty((P, 2, N), dtype=torch.complex64, device=self.u0.device)
-            self._keep.append(work)
+            work = self.work
+            if work is None:
+                work = torch.empty((P, 2, N), dtype=torch.complex64, device=self.u0.device)
+                self._keep.append(work)
+            assert work.shape[0] >= P and work.stride(0) == 2 * N and work.stride(-1) == 1
             s.work = _lib.ptr(work)
             s.work_pstride = 2 * N
         if self.exit is not None:
@@ -352,7 +356,8 @@ class BandScanSpec:
             s.ckpt_every = self.ckpt_every
         if self.rhs is not None:
             r = self.rhs if f.b_sign == 1.0 else self.rhs * f.b_sign
-            r = r.contiguous()
+            if r.stride(-1) != 1:
+                r = r.contiguous()
             self._keep.append(r)
             s.rhs = _lib.ptr(r)
             s.rhs_pstride = 0 if r.shape[0] == 1 else r.stride(0)
@@ -379,3 +384,33 @@ def sweep_shapes(w: int):
     if w <= 8:
         shapes += [(1, 1), (2, 1)]
     return shapes
+
+
+def residual(bf: "BandFactorizer", C: torch.Tensor, parts: Sequence[torch.Tensor], r: torch.Tensor,
+             stream=None) -> torch.Tensor:
+    """r_s = 2 v_{s-1} - A_s (v_{s-1} + v_s) in FP64 (tp_band_residual) for every step
+    of the factorizer's operator (its own FP64 blocks and C, an (N, L) complex128
+    view of any strides), v the sum of the complex64 plane sets ``parts`` (each
+    (P, L+1, N), the same strides); r: complex64 (P, L, N) with unit last stride.
+    A complex64 sweep of the same factors on r (rhs_steps = L) gives the correction
+    of the planes: refinement of the forward stepping (sigma = 2 modes)."""
+    pl = bf.plan
+    if bf.source is not None or pl.sigma or pl.b_sign != 1.0:
+        raise ValueError("residuals are built for the forward factorizer")
+    if not 1 <= len(parts) <= 3:
+        raise ValueError("one to three plane sets")
+    u = parts[0]
+    P, L = u.shape[0], bf.L
+    N = bf.nx * bf.ny
+    for x in parts:
+        assert x.dtype == torch.complex64 and x.shape[0] == P and x.shape[1] >= L + 1 and x.shape[-1] == N
+        assert x.stride() == u.stride() and x.stride(-1) == 1
+    assert C.dtype == torch.complex128 and tuple(C.shape) == (N, L)
+    assert r.dtype == torch.complex64 and r.shape[0] == P and r.shape[1] >= L and r.stride(-1) == 1
+    ptrs = [_lib.ptr(x) for x in parts] + [0] * (3 - len(parts))
+    _lib.check(_lib.load().tp_band_residual(ptrs[0], ptrs[1], ptrs[2], u.stride(0), u.stride(1), P, bf.nx, bf.ny, L,
+                                            _lib.ptr(C), C.stride(0), C.stride(1), L, _lib.ptr(bf.tri),
+                                            _lib.ptr(bf.cpl), pl.lhs_sign, pl.conj_c, pl.lhs_col0, pl.col_step, 0,
+                                            _lib.ptr(r), r.stride(0), r.stride(1), _lib.stream_handle(stream)),
+               "tp_band_residual")
+    return r

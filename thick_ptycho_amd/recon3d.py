@@ -34,10 +34,20 @@ from .forward_model import band_engine as be
 class Reconstruction3D:
     def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
                  batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None,
-                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6, exit_data=None):
-        """``sqrt_i`` (P, nx, ny): measured far-field amplitudes; or ``sqrt_i=None`` with
-        ``exit_data`` (P, nx, ny) complex exit waves and the loss sum |u_L - exit_data|^2
-        (the reference's non phase-retrieval data; iterations then run eagerly)."""
+                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6, exit_data=None, refine: int = 0):
+        """``sqrt_i`` (P, nx, ny): measured far-field amplitudes (float64 amplitudes are
+        kept as float32 high and low parts); or ``sqrt_i=None`` with ``exit_data``
+        (P, nx, ny) complex exit waves and the loss sum |u_L - exit_data|^2 (the
+        reference's non phase-retrieval data; iterations then run eagerly).
+
+        ``refine`` (0, 1 or 2; banded forward stepping with amplitude data): refinement
+        steps of the exit plane before the loss seed, each an FP64 residual of the
+        recurrence (tp_band_residual) and a complex64 correction sweep of the same
+        factors, with the seed then formed in FP64 (tp_amplitude_seed2d_refined).
+        Where the model's far field lies many orders of magnitude below its peak the
+        seed's phase follows the exit plane's rounding; two steps bring the exit
+        plane to FP64 accuracy, at the cost of two residual passes and two more band
+        sweeps per gradient."""
         dev = device or torch.device("cuda", torch.cuda.current_device())
         self.dev = dev
         n0 = np.asarray(n0)
@@ -50,11 +60,21 @@ class Reconstruction3D:
         self.P = P
         self.batch = batch or P
         self.n = torch.tensor(n0, dtype=torch.complex128, device=dev)
-        self.probes = torch.as_tensor(probes).to(dev, torch.complex64).reshape(P, -1).contiguous()
+        pr = torch.as_tensor(probes).to(dev).reshape(P, -1)
+        self.probes = pr.to(torch.complex64).contiguous()
+        # complex128 probes: their complex64 rounding error starts the first refinement sweep
+        self.probes_lo = (pr - self.probes.to(pr.dtype)).to(torch.complex64).contiguous() \
+            if pr.dtype == torch.complex128 else None
         if (sqrt_i is None) == (exit_data is None):
             raise ValueError("give either sqrt_i (intensity data) or exit_data (exit waves)")
-        self.sqrt_i = None if sqrt_i is None else \
-            torch.as_tensor(sqrt_i).to(dev, torch.float32).reshape(P, nx, ny).contiguous()
+        self.sqrt_i_lo = None
+        if sqrt_i is None:
+            self.sqrt_i = None
+        else:
+            a = torch.as_tensor(sqrt_i).to(dev).reshape(P, nx, ny)
+            self.sqrt_i = a.to(torch.float32).contiguous()
+            if a.dtype == torch.float64:
+                self.sqrt_i_lo = (a - self.sqrt_i.double()).to(torch.float32).contiguous()
         self.exit_data = None if exit_data is None else \
             torch.as_tensor(exit_data).to(dev, torch.complex64).reshape(P, nx * ny).contiguous()
         A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
@@ -78,6 +98,19 @@ class Reconstruction3D:
         self.graph = None
         if forward not in ("band", "krylov"):
             raise ValueError("forward must be 'band' or 'krylov'")
+        if refine not in (0, 1, 2):
+            raise ValueError("refine must be 0, 1 or 2")
+        if refine and (forward != "band" or self.sqrt_i is None):
+            raise ValueError("refinement is built for the banded forward stepping with amplitude data")
+        self.refine = refine
+        if refine:
+            lib = _lib.load()
+            self.CT = torch.empty((self.L, self.N), dtype=torch.complex128, device=dev)  # C by step: coalesced reads
+            self.corr = torch.empty_like(self.u) if refine == 2 else None
+            self.xparts = torch.empty((3, Pb, self.N), dtype=torch.complex64, device=dev)
+            self.sweep_work = torch.empty((Pb, 2, self.N), dtype=torch.complex64, device=dev)
+            nbytes = int(lib.tp_amplitude_seed2d_refined_work_bytes(nx, ny, Pb))
+            self.seed_work = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=dev)
         self.forward = forward
         self.krylov = None
         self.krylov_stats = []
@@ -117,7 +150,9 @@ class Reconstruction3D:
                 be.sweep([be.BandScanSpec(fwd, self.probes[b0:b1], out=u)], self.ppb, schedule=self.schedule)
             ex = self.exit[:nb]
             ex.copy_(u[:, self.L])
-            if self.exit_data is not None:  # J = sum |u_L - d|^2: seed u_L - d
+            if self.refine and seed is None:
+                self._refined_seed(b0, b1, u)
+            elif self.exit_data is not None:  # J = sum |u_L - d|^2: seed u_L - d
                 torch.sub(ex, self.exit_data[b0:b1], out=self.seed[:nb, 0])
                 self.loss[b0:b1] = (self.seed[:nb, 0].abs() ** 2).sum(dim=1)
             else:
@@ -134,6 +169,40 @@ class Reconstruction3D:
         self.grad[:, :-1] += Gt
         self.grad[:, 1:] += Gt
         self.grad.mul_((self.dz / 4) * (self.k / 1j)).conj_physical_().mul_(2)
+
+    def _refined_seed(self, b0: int, b1: int, u: torch.Tensor) -> None:
+        """Loss seed and per-probe loss of the refined exit plane u_L + e1_L (+ e2_L):
+        FP64 residuals of the forward recurrence at the iterate, complex64 correction
+        sweeps of the forward factors (the first started from the probes' complex64
+        rounding error when they were given in complex128), the seed in FP64.  The residuals use lam as scratch (it is written by the adjoint sweep
+        afterwards)."""
+        lib = _lib.load()
+        nb, L = b1 - b0, self.L
+        fac = self.fwd.fac
+        self.CT.copy_(self.C.transpose(0, 1))
+        Cv = self.CT.transpose(0, 1)
+        r = self.lam[:nb, :L]
+        X = self.xparts[:, :nb]
+        X[0].copy_(u[:, L])
+        e0 = self.zero[:nb] if self.probes_lo is None else self.probes_lo[b0:b1]
+        be.residual(self.fwd, Cv, [u], r)
+        if self.refine == 1:
+            be.sweep([be.BandScanSpec(fac, e0, exit=X[1], rhs=r, rhs_steps=L, work=self.sweep_work)],
+                     self.ppb, schedule=self.schedule)
+            parts = (X[0], X[1], None)
+        else:
+            e1 = self.corr[:nb]
+            be.sweep([be.BandScanSpec(fac, e0, out=e1, rhs=r, rhs_steps=L)], self.ppb, schedule=self.schedule)
+            be.residual(self.fwd, Cv, [u, e1], r)
+            X[1].copy_(e1[:, L])
+            be.sweep([be.BandScanSpec(fac, self.zero[:nb], exit=X[2], rhs=r, rhs_steps=L, work=self.sweep_work)],
+                     self.ppb, schedule=self.schedule)
+            parts = (X[0], X[1], X[2])
+        lo = None if self.sqrt_i_lo is None else self.sqrt_i_lo[b0:b1]
+        _lib.check(lib.tp_amplitude_seed2d_refined(*[_lib.ptr(x) for x in parts], self.N, _lib.ptr(self.sqrt_i[b0:b1]),
+                                                   _lib.ptr(lo), self.nx, self.ny, nb, 1.0, _lib.ptr(self.seed[:nb]),
+                                                   _lib.ptr(self.loss[b0:b1]), _lib.ptr(self.seed_work),
+                                                   _lib.stream_handle()), "tp_amplitude_seed2d_refined")
 
     def grad_object(self) -> torch.Tensor:
         return self.grad.reshape(self.nx, self.ny, self.nz)
@@ -214,6 +283,56 @@ class Reconstruction3D:
 
     def object(self) -> np.ndarray:
         return self.n.cpu().numpy()
+
+
+def simulate_exit_3d(n, probes, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet", refine: int = 2,
+                     batch: Optional[int] = None, device=None) -> torch.Tensor:
+    """Exit planes of banded-LU forward stepping of every probe through ``n``
+    (nx, ny, nz), refined: complex64 (1 + refine, P, nx*ny) on the GPU whose sum is
+    u_L to FP64 accuracy after two refinement steps (the complex64 plane, then
+    the exit parts of the correction sweeps driven by FP64 residuals of the
+    recurrence, tp_band_residual; complex128 probes start the first correction
+    from their complex64 rounding error).  Keeps every plane of a batch of probes."""
+    if refine not in (0, 1, 2):
+        raise ValueError("refine must be 0, 1 or 2")
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    n = np.asarray(n)
+    nx, ny, nz = n.shape
+    N, L = nx * ny, nz - 1
+    P = probes.shape[0]
+    batch = batch or P
+    coef = (k / 1j) * (n.reshape(N, nz) - 1)
+    C = torch.tensor((dz / 2) * (coef[:, :-1] + coef[:, 1:]) / 2, dtype=torch.complex128, device=dev)
+    A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
+    w = be.factor(C, A, B, "forward").w
+    bf = be.BandFactorizer(A, B, "forward", L, w, dev)
+    fac = bf.run(C)
+    bf.check()
+    pr = torch.as_tensor(probes).to(dev).reshape(P, N)
+    u0 = pr.to(torch.complex64).contiguous()
+    lo = (pr - u0.to(pr.dtype)).to(torch.complex64).contiguous() if pr.dtype == torch.complex128 else None
+    out = torch.empty((1 + refine, P, N), dtype=torch.complex64, device=dev)
+    Pb = min(batch, P)
+    u = torch.empty((Pb, L + 1, N), dtype=torch.complex64, device=dev)
+    e1 = torch.empty_like(u) if refine == 2 else None
+    r = torch.empty((Pb, L, N), dtype=torch.complex64, device=dev) if refine else None
+    zero = torch.zeros((Pb, N), dtype=torch.complex64, device=dev)
+    for b0 in range(0, P, batch):
+        b1 = min(b0 + batch, P)
+        nb = b1 - b0
+        be.sweep([be.BandScanSpec(fac, u0[b0:b1], out=u[:nb])])
+        out[0, b0:b1].copy_(u[:nb, L])
+        if refine:
+            e0 = zero[:nb] if lo is None else lo[b0:b1]
+            be.residual(bf, C, [u[:nb]], r[:nb])
+            if refine == 1:
+                be.sweep([be.BandScanSpec(fac, e0, exit=out[1, b0:b1], rhs=r[:nb], rhs_steps=L)])
+            else:
+                be.sweep([be.BandScanSpec(fac, e0, out=e1[:nb], rhs=r[:nb], rhs_steps=L)])
+                out[1, b0:b1].copy_(e1[:nb, L])
+                be.residual(bf, C, [u[:nb], e1[:nb]], r[:nb])
+                be.sweep([be.BandScanSpec(fac, zero[:nb], exit=out[2, b0:b1], rhs=r[:nb], rhs_steps=L)])
+    return out
 
 
 def simulate_intensity_3d(n, probes, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
