@@ -272,3 +272,88 @@ def test_half_chunk_scans_in_one_launch(gpu):
         torch.testing.assert_close(mixed, apart, rtol=0, atol=0)
     for i, ref in refs:
         assert rel_l2(outs[i][0].cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("mode", ["forward", "exact_adjoint"])
+@pytest.mark.parametrize("widths", [(1024, 384), (1536, 768)])
+def test_half_chunk_modes_and_widths(gpu, mode, widths):
+    """Half-chunk scans beside full-chunk ones in one persistent launch, for the
+    sigma mode too (exact adjoint: six coefficient arrays per row in the narrow
+    scan's LDS prefix) and for a 32-row launch with a 16-row scan (1536 + 768:
+    probes per workgroup capped at 2): bitwise equal to the two chunk widths
+    launched apart, and each scan against the FP64 oracle."""
+    rng = np.random.default_rng(17)
+    nz, P, k = 40, 6, 2 * np.pi
+    L = nz - 1
+    specs, outs, refs = [], [], []
+    for nx in widths:
+        g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=2.0, dz=4.0, k=k)
+        n = random_object(nx, nz, rng, amp=1e-2)
+        A, B = operators.step_operators_1d(nx, k, 2.0, 4.0, "impedance")
+        C = torch.from_numpy(pwe.object_contribution(g, n)).to("cuda")
+        fac = tri_engine.factor(C, A, B, mode, chunk=operators.chunk_for(nx))
+        probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+        if mode == "forward":
+            u0, rhs = torch.from_numpy(probes).to("cuda", torch.complex64), None
+            ref = pwe.solve_batched(g, pwe.build_lu_cache(g, "impedance", n, "forward"), probes, "forward", keep="exit")
+        else:  # lambda_1 of the exact adjoint with the probes as the seed
+            u0 = torch.zeros((P, nx), dtype=torch.complex64, device="cuda")
+            rhs = torch.zeros((P, L, nx), dtype=torch.complex64, device="cuda")
+            rhs[:, 0] = torch.from_numpy(probes).to("cuda", torch.complex64)
+            ref = exact_adjoint_states(g, "impedance", n, probes)[:, :, 1]
+        specs.append((fac, u0, rhs))
+        outs.append((torch.empty((P, nx), dtype=torch.complex64, device="cuda"),
+                     torch.empty((P, nx), dtype=torch.complex64, device="cuda")))
+        refs.append(ref)
+    chunks = [f.chunk for f, _, _ in specs]
+    assert chunks[1] * 2 == chunks[0], chunks
+    tri_engine.sweep([tri_engine.TriScanSpec(f, u0, rhs=r, exit=o[0]) for (f, u0, r), o in zip(specs, outs)],
+                     probes_per_block=4)
+    for i, (f, u0, r) in enumerate(specs):
+        tri_engine.sweep([tri_engine.TriScanSpec(f, u0, rhs=r, exit=outs[i][1])], probes_per_block=4)
+    for (mixed, apart), ref, nx in zip(outs, refs, widths):
+        torch.testing.assert_close(mixed, apart, rtol=0, atol=0)
+        err = rel_l2(mixed.cpu().numpy(), ref)
+        print(f"{mode} {widths}: {nx}-wide scan rel-L2 {err:.2e}")
+        assert err < TOL
+
+
+def test_zero_pivot_raises(gpu):
+    """A slice whose first pivot is exactly zero: solve() raises RuntimeError
+    (the reference's splu raises on an exactly singular factor) instead of
+    returning inf/NaN fields, and so do the factors' own check and
+    tri_engine.check_pivots."""
+    nx, nz, P = 128, 6, 2
+    rng = np.random.default_rng(3)
+    probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+    s = _solver(nx, nz, 2.0, 4.0, 2 * np.pi, probes, "dirichlet")
+    A, B = s.pwe_finite_differences.A, s.pwe_finite_differences.B
+    C = np.zeros((nx, nz - 1), dtype=complex)
+    C[0, 2] = A.diag[0]  # forward LHS of step 3: A0 - diag(C[:, 2]), first pivot A0[0, 0] - C[0, 2] = 0
+    s.object_contribution_device = lambda n, scan_idx=0: torch.from_numpy(C).to("cuda")
+    with pytest.raises(RuntimeError, match="zero pivot"):
+        s.solve(n=np.ones((nx, nz), dtype=complex))
+    fac = tri_engine.factor(torch.from_numpy(C).to("cuda"), A, B, "forward")
+    with pytest.raises(RuntimeError, match="zero pivot"):
+        fac.check()
+    with pytest.raises(RuntimeError, match="zero pivot"):
+        tri_engine.check_pivots([fac])
+    ok = tri_engine.factor(torch.zeros((nx, nz - 1), dtype=torch.complex128, device="cuda"), A, B, "forward")
+    tri_engine.check_pivots([ok])
+
+
+@pytest.mark.parametrize("chunk_planes", [1, 3, 7])
+def test_host_handover_staged_above_the_pinned_limit(gpu, monkeypatch, chunk_planes):
+    """Results above PINNED_RESULT_MAX_BYTES go through two page-locked staging
+    chunks into ordinary memory: equal to numpy's transpose and widening of the
+    device planes, for chunkings that split the leading axes unevenly."""
+    from thick_ptycho_amd.forward_model import base_solver
+
+    nz, n = 9, 70
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(2, 1, 5, nz, n, dtype=torch.complex64, device="cuda", generator=g)
+    monkeypatch.setattr(base_solver, "PINNED_RESULT_MAX_BYTES", 0)
+    monkeypatch.setattr(base_solver, "HOST_CHUNK_BYTES", 16 * nz * n * chunk_planes)
+    h = base_solver.host_complex128(x.transpose(-1, -2))
+    assert h.dtype == np.complex128 and h.shape == (2, 1, 5, n, nz) and h.flags["C_CONTIGUOUS"]
+    np.testing.assert_array_equal(h, x.cpu().numpy().transpose(0, 1, 2, 4, 3).astype(np.complex128))

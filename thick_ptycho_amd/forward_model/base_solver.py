@@ -26,6 +26,28 @@ from .. import _lib
 from . import ms_engine
 
 HOST_CHUNK_BYTES = 1 << 28  # complex128 bytes converted on the device per host copy
+# Results up to this size are returned in page-locked memory of their own (torch's
+# host allocator caches such blocks, rounded up to a power of two, for reuse after
+# the array is freed); larger ones are staged through two page-locked chunk
+# buffers into ordinary memory, so repeated large solves pin at most 2 chunks.
+PINNED_RESULT_MAX_BYTES = 1 << 30
+_STAGE: list = []
+_COPY_THREADS = 8
+
+
+def _stage_buffers(nbytes: int) -> list:
+    """Two page-locked complex128 staging buffers of at least ``nbytes`` each, kept for reuse."""
+    global _STAGE
+    if not _STAGE or _STAGE[0].numel() * 16 < nbytes:
+        _STAGE = [torch.empty(max(1, nbytes // 16), dtype=torch.complex128, pin_memory=True) for _ in range(2)]
+    return _STAGE
+
+
+def _parallel_copy(dst: np.ndarray, src: np.ndarray, pool) -> None:
+    """dst[...] = src over the leading axis in a few threads (numpy releases the GIL)."""
+    n = dst.shape[0]
+    step = max(1, -(-n // _COPY_THREADS))
+    list(pool.map(lambda a: np.copyto(dst[a:a + step], src[a:a + step]), range(0, n, step)))
 
 
 def host_complex128(u: torch.Tensor) -> np.ndarray:
@@ -33,27 +55,54 @@ def host_complex128(u: torch.Tensor) -> np.ndarray:
 
     The reference returns complex128 fields on the host (base/base_solver.py:180-188).
     The transpose and the complex64 -> complex128 widening run on the device,
-    a chunk of the leading axes at a time, and each chunk is copied into
-    page-locked host memory (the device-to-host copy then runs at the link's DMA
-    rate instead of through pageable staging and a single-threaded host cast);
-    where page-locked memory cannot be had the chunks go into ordinary memory.
-    torch's host allocator caches page-locked blocks (sizes rounded up to a power
-    of two) for reuse once the returned array is freed.
+    a chunk of the leading axes at a time.  Results up to PINNED_RESULT_MAX_BYTES
+    are copied straight into a page-locked result (the device-to-host copy runs
+    at the link's DMA rate); larger ones go through two page-locked staging
+    chunks, alternately filled by the device and copied by host threads into an
+    ordinary array, so that pinned host memory stays bounded.  Where page-locked
+    memory cannot be had the chunks go into ordinary memory.
     """
     shape = tuple(u.shape)
-    try:
-        host = torch.empty(shape, dtype=torch.complex128, pin_memory=u.is_cuda)
-    except RuntimeError:
-        host = torch.empty(shape, dtype=torch.complex128)
-    if u.numel() == 0:
-        return host.numpy()
+    nbytes = 16 * int(np.prod(shape)) if shape else 16
     inner = shape[-2:] if len(shape) >= 2 else shape
-    lead = u.reshape(-1, *inner)
-    dst = host.view(-1, *inner)
     per = max(1, HOST_CHUNK_BYTES // (16 * max(1, int(np.prod(inner)))))
-    for i in range(0, lead.shape[0], per):
-        dst[i:i + per].copy_(lead[i:i + per].to(torch.complex128))
-    return host.numpy()
+    if not u.is_cuda or nbytes <= PINNED_RESULT_MAX_BYTES or u.numel() == 0:
+        try:
+            host = torch.empty(shape, dtype=torch.complex128, pin_memory=u.is_cuda)
+        except RuntimeError:
+            host = torch.empty(shape, dtype=torch.complex128)
+        if u.numel() == 0:
+            return host.numpy()
+        lead = u.reshape(-1, *inner)
+        dst = host.view(-1, *inner)
+        for i in range(0, lead.shape[0], per):
+            dst[i:i + per].copy_(lead[i:i + per].to(torch.complex128))
+        return host.numpy()
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = np.empty(shape, dtype=np.complex128)
+    lead = u.reshape(-1, *inner)
+    dst = out.reshape(-1, *inner)
+    plane = max(1, int(np.prod(inner)))
+    stage = _stage_buffers(16 * per * plane)
+    done = [torch.cuda.Event(), torch.cuda.Event()]
+    pending = None  # (buffer index, first row, rows) copied to the host after the next chunk is issued
+    with ThreadPoolExecutor(_COPY_THREADS) as pool:
+        for k, i in enumerate(range(0, lead.shape[0], per)):
+            rows = min(per, lead.shape[0] - i)
+            b = k % 2
+            buf = stage[b][: rows * plane].view(rows, *inner)
+            buf.copy_(lead[i:i + rows].to(torch.complex128), non_blocking=True)
+            done[b].record()
+            if pending is not None:
+                pb, pi, pr = pending
+                done[pb].synchronize()
+                _parallel_copy(dst[pi:pi + pr], stage[pb][: pr * plane].view(pr, *inner).numpy(), pool)
+            pending = (b, i, rows)
+        pb, pi, pr = pending
+        done[pb].synchronize()
+        _parallel_copy(dst[pi:pi + pr], stage[pb][: pr * plane].view(pr, *inner).numpy(), pool)
+    return out
 
 
 def _setup_log(verbose: bool):

@@ -204,8 +204,10 @@ class PWEIterativeLUSolver(BasePWESolver):
             self._solve_reduced(n, mode, pr, rhs, per_probe_rhs, out, keep, plain)
             return out if keep == "all" else out[..., 0, :]
         specs = []
+        self._last_factors = []
         for p in range(self.num_projections):
             fac = self._get_or_construct_cache(n=n, mode=mode, proj_idx=p).factors
+            self._last_factors.append(fac)
             for a in range(A):
                 u0 = pr[a]
                 if keep == "all":
@@ -233,6 +235,7 @@ class PWEIterativeLUSolver(BasePWESolver):
         wide = self.waves > 1
         t0 = time.time()
         windows = self._reduced_windows()
+        self._last_factors = []
         ss = self.simulation_space
         device_windows = (windows is not None and not getattr(ss, "exact_ref_coeff", False)
                           and getattr(ss, "empty_space_px", 0) == 0)
@@ -256,6 +259,7 @@ class PWEIterativeLUSolver(BasePWESolver):
                 facs = tri_engine.factor_many(jobs, self.chunk, out=[coef[s] for s in range(i, min(i + 8, S))],
                                               waves=self.waves if wide else None)
                 fac = fac or facs[0]
+                self._last_factors += facs
             whole = tri_engine.TriFactors(coef.reshape(-1), fac.n, L, self.chunk, mode, fac.b_sign, fac.status,
                                           fac.sigma, keep=[coef], waves=self.waves, wide=wide)
             if plain:  # the plain form shares factors within a scan: one scan per window
@@ -301,5 +305,6 @@ class PWEIterativeLUSolver(BasePWESolver):
         (Proj, A, S, *effective_shape) on the host."""
         assert mode in {"forward", "adjoint", "reverse"}, f"Invalid mode: {mode!r}"
         u = self.solve_device(n=n, mode=mode, rhs_block=rhs_block, probes=probes, keep="all")
+        tri_engine.check_pivots(getattr(self, "_last_factors", []))  # splu raises on a singular slice
         u = host_complex128(u.transpose(-1, -2))
         return u.reshape(self.num_projections, self.num_angles, self.num_probes, *self.effective_shape)

@@ -46,6 +46,14 @@ class TriFactors:
             raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
 
 
+def check_pivots(facs) -> None:
+    """Raise RuntimeError if any of the factor sets hit an exactly zero pivot
+    (one synchronisation for all of them); the reference's splu raises there."""
+    st = [f.status for f in facs if isinstance(f, TriFactors)]
+    if st and int(torch.stack([x.reshape(()) for x in st]).max().item()) != 0:
+        raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
+
+
 # The per-mode plan and its device copy of the base diagonals depend only on the
 # step operators (A0, B0), the mode and L -- not on the object -- so repeated
 # factorisations (a new object every pass or iteration) reuse them, like the

@@ -221,6 +221,7 @@ def amplitude_gradient(n, probes, sqrt_i, k: float, dx: float, dz: float, bc="di
     Gn[:, :-1] += Gt
     Gn[:, 1:] += Gt
     Gn *= (dz / 4) * (k / 1j)
+    tri_engine.check_pivots([sw.fwd, sw.adj])  # with the loss read below: one synchronisation point
     return float(loss.double().sum().item()), 2 * torch.conj(Gn)
 
 
