@@ -54,6 +54,8 @@ def main() -> int:
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D (config 3) banded-LU and learned lines")
     ap.add_argument("--no-recon", action="store_true", help="skip the 3D reconstruction loop (256x256x128)")
     ap.add_argument("--recon-iters", type=int, default=50)
+    ap.add_argument("--no-fixture", action="store_true",
+                    help="skip the 3D reconstruction's 5-iteration comparisons with the FP64 fixture")
     ap.add_argument("--cpu-probes", type=int, default=128)
     ap.add_argument("--cpu-probes-per-proc", type=int, default=32)
     ap.add_argument("--ppb", default="auto", help="probes per workgroup of the persistent sweep, or 'auto'")
@@ -379,7 +381,7 @@ def main() -> int:
     if not args.no_3d:
         extras["config3"], cnn = bench_3d(dev, args.steps)
     if not args.no_recon:
-        extras["config4"] = bench_recon(dev, args.recon_iters, cnn)
+        extras["config4"] = bench_recon(dev, args.recon_iters, cnn, fixture=not args.no_fixture)
 
     cpu, cpu_par = None, None
     if not args.no_cpu:
@@ -525,7 +527,7 @@ def bench_3d(dev, steps: int) -> dict:
 RECON_FIXTURE = os.path.join(ROOT, "tests", "golden", "recon3d_256x256x128_fp64.npz")
 
 
-def bench_recon(dev, iters: int, cnn=None) -> dict:
+def bench_recon(dev, iters: int, cnn=None, fixture: bool = True) -> dict:
     """3D reconstruction loop on 256 x 256 x 128 with 400 probes: intensities of the
     seeded true object simulated on the GPU, start n0 = 1, ``iters`` steepest-descent
     iterations (forward + exact adjoint banded-LU stepping of every probe, 2D far-field
@@ -584,7 +586,7 @@ def bench_recon(dev, iters: int, cnn=None) -> dict:
     })
     log(f"3D reconstruction: {iters} iterations in {dt:.2f} s ({res['iteration_ms']:.1f} ms each), "
         f"loss {out['loss'][0]:.4e} -> {out['loss'][-1]:.4e}, object error vs truth {res['object_error_vs_truth']:.4f}")
-    if os.path.exists(RECON_FIXTURE):
+    if fixture and os.path.exists(RECON_FIXTURE):
         fx = np.load(RECON_FIXTURE)
         rc.reset(n0)
         o5 = rc.run(5, alpha=float(fx["alpha"]), graph=True)
@@ -602,7 +604,7 @@ def bench_recon(dev, iters: int, cnn=None) -> dict:
         log(f"vs FP64 CPU reconstruction (5 iterations): {res['fp64_5iter']}")
     hist_band = out["loss"]
     del rc
-    if os.path.exists(RECON_FIXTURE):
+    if fixture and os.path.exists(RECON_FIXTURE):
         # refined exit plane (two FP64-residual correction sweeps) and FP64 loss seed,
         # with the FP64 intensities of refined exit planes as data (complex128 probes)
         fx = np.load(RECON_FIXTURE)

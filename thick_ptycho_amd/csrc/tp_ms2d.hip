@@ -243,6 +243,46 @@ __global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass3_kernel(const Ms2dPas
   }
 }
 
+// pass 3 of slice z fused with pass 1 of slice z + 1 (both run along the rows a):
+// psi_{z+1} = IFFT_b(spectrum row), optionally stored (natural order, to
+// `store`), then y = FFT_b(psi_{z+1} * T_{z+1}) stored transposed for the next
+// pass 2.  The plane between the two never goes through HBM unless it is kept.
+template <int R>
+__global__ __launch_bounds__(64 * M2_WAVES) void ms2d_pass31_kernel(const Ms2dPass a, float2* store,
+                                                                   int64_t store_pstride) {
+  constexpr int L = 64 * R;
+  constexpr int RPW = m2_rows<R>();
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[M2_WAVES][2][ms_bufn<R, false>()];
+  __shared__ float2 tile[RPW * tile_ld(L)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  build_twiddles(tw, L);
+  const int groups = (a.rows + RPW - 1) / RPW;
+  const int p = blockIdx.x / groups, row0 = (blockIdx.x % groups) * RPW;
+  const int nr = min(RPW, a.rows - row0);
+  __syncthreads();
+  for (int r = wave; r < nr; r += M2_WAVES) {
+    const float2* src = a.in + (size_t)p * a.in_pstride + (size_t)(row0 + r) * L;
+    const float2* t = a.T + (size_t)(row0 + r) * L;
+    float2 x[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = src[spec_index<R>(lane, m)];
+    fft_any<R, +1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
+    if (store) {
+      float2* dst = store + (size_t)p * store_pstride + (size_t)(row0 + r) * L;
+#pragma unroll
+      for (int m = 0; m < R; ++m) dst[lane + 64 * m] = x[m];
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = cmul(x[m], t[lane + 64 * m]);
+    fft_any<R, -1, false>(x, buf[wave][0], buf[wave][1], tw, lane);
+#pragma unroll
+    for (int m = 0; m < R; ++m) tile[r * tile_ld(L) + spec_index<R>(lane, m)] = x[m];
+  }
+  __syncthreads();
+  store_tile_t(tile, a.out, a.out_pstride, p, a.rows, L, row0, nr, threadIdx.x, blockDim.x);
+}
+
 __global__ void ms2d_kernel_kernel(int A, int B, double da, double db, double k, double dz, float2* __restrict__ H) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)A * B) return;
@@ -282,6 +322,12 @@ template <int R>
 static void launch_p3(const Ms2dPass& a, hipStream_t st) {
   const int groups = (a.rows + m2_rows<R>() - 1) / m2_rows<R>();
   hipLaunchKernelGGL((ms2d_pass3_kernel<R>), dim3(a.P * groups), dim3(64 * M2_WAVES), 0, st, a);
+}
+template <int R>
+static void launch_p31(const Ms2dPass& a, float2* store, int64_t store_pstride, hipStream_t st) {
+  const int groups = (a.rows + m2_rows<R>() - 1) / m2_rows<R>();
+  hipLaunchKernelGGL((ms2d_pass31_kernel<R>), dim3(a.P * groups), dim3(64 * M2_WAVES), 0, st, a, store,
+                     store_pstride);
 }
 
 #define TP_M2_DISPATCH(L, CALL)                      \
@@ -323,26 +369,32 @@ extern "C" int tp_ms2d_sweep(const void* d_T, const void* d_H, const void* d_u0,
   const int64_t N = (int64_t)nx * ny, PN = (int64_t)probes * N;
   float2* w1 = (float2*)d_work;  // transposed intermediate, plane p at w1 + p N ([ny][nx])
   float2* w2 = w1 + PN;          // spectrum along y, [nx][ny]
-  float2* psi = w2 + PN;         // current plane when no full field is written
+  float2* psi = w2 + PN;         // last plane when no full field is written
   float2* out = (float2*)d_out;
   const int64_t ops = (int64_t)(steps + 1) * N;  // full-field plane stride of one probe
   const dim3 cg(64, (unsigned)probes);
-  const float2* cur = (const float2*)d_u0;
-  int64_t cps = N;
-  if (out) hipLaunchKernelGGL(ms2d_copy_kernel, cg, dim3(256), 0, st, cur, N, out, ops, N);  // plane 0
+  if (out) hipLaunchKernelGGL(ms2d_copy_kernel, cg, dim3(256), 0, st, (const float2*)d_u0, N, out, ops, N);
+  // Two passes per slice: pass 2 (along x) and pass 3 of the slice fused with
+  // pass 1 of the next (both along y); pass 1 runs alone before the first slice
+  // and pass 3 alone after the last.
+  Ms2dPass a1{(const float2*)d_u0, w1, nullptr, (const float2*)d_T, nullptr, probes, nx, ny, 0, N, N};
+  TP_M2_DISPATCH(ny, launch_p1<RV>(a1, st));
   for (int z = 0; z < steps; ++z) {
-    float2* nxt = out ? out + (size_t)(z + 1) * N : psi;
-    const int64_t nps = out ? ops : N;
-    Ms2dPass a1{cur, w1, nullptr, (const float2*)d_T + (size_t)z * N, nullptr, probes, nx, ny, 0, cps, N};
-    TP_M2_DISPATCH(ny, launch_p1<RV>(a1, st));
     Ms2dPass a2{w1, w2, nullptr, nullptr, (const float2*)d_H, probes, ny, nx, conj_h, N, N};
     TP_M2_DISPATCH(nx, (launch_p2<RV, false>(a2, st)));
-    Ms2dPass a3{w2, nxt, nullptr, nullptr, nullptr, probes, nx, ny, 0, N, nps};
-    TP_M2_DISPATCH(ny, launch_p3<RV>(a3, st));
-    cur = nxt;
-    cps = nps;
+    if (z + 1 < steps) {
+      Ms2dPass a31{w2, w1, nullptr, (const float2*)d_T + (size_t)(z + 1) * N, nullptr, probes, nx, ny, 0, N, N};
+      float2* keep = out ? out + (size_t)(z + 1) * N : nullptr;
+      TP_M2_DISPATCH(ny, launch_p31<RV>(a31, keep, ops, st));
+    } else {
+      float2* last = out ? out + (size_t)(z + 1) * N : psi;
+      Ms2dPass a3{w2, last, nullptr, nullptr, nullptr, probes, nx, ny, 0, N, out ? ops : N};
+      TP_M2_DISPATCH(ny, launch_p3<RV>(a3, st));
+      if (d_exit)
+        hipLaunchKernelGGL(ms2d_copy_kernel, cg, dim3(256), 0, st, (const float2*)last, out ? ops : N,
+                           (float2*)d_exit, N, N);
+    }
   }
-  if (d_exit) hipLaunchKernelGGL(ms2d_copy_kernel, cg, dim3(256), 0, st, cur, cps, (float2*)d_exit, N, N);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }
