@@ -505,23 +505,82 @@ def bench_3d(dev, steps: int) -> dict:
     res["cnn_mfma_frac_bf16_dense"] = flops / t_cnn / 2.5e15
     log(f"CNN apply ({P} planes): {t_cnn * 1e3:.3f} ms, {res['cnn_tflops']:.1f} TFLOP/s")
 
-    # learned stepping: train on banded-LU pairs (seeds 31-32), step the config-3 object
-    objs = [workloads.ellipsoid_object_3d(nx, ny, 64, 32, np.random.default_rng(sd)) for sd in (31, 32)]
-    pairs = tl.generate_pairs(objs, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, per_slice=16, slices=16, seed=33)
+    # learned stepping: train on 20480 banded-LU residual/correction pairs of two held-out
+    # objects (seeds 31-32, 160 slices x 64 planes), step the config-3 object by BiCGSTAB to 1e-6
+    objs = [workloads.ellipsoid_object_3d(nx, ny, 256, 32, np.random.default_rng(sd)) for sd in (31, 32)]
     t0 = time.perf_counter()
-    losses = cnn.train(pairs, steps=300, batch=16, lr=2e-3)
+    pairs = tl.generate_pairs(objs, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, per_slice=64, slices=160, seed=33)
     torch.cuda.synchronize()
+    res["cnn_pairs"] = pairs.count
+    res["cnn_pairs_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = cnn.train(pairs, steps=1500, batch=32, lr=2e-3)
+    torch.cuda.synchronize()
+    del pairs
     res["cnn_train_s"] = time.perf_counter() - t0
     res["cnn_train_loss"] = losses[-1]
-    for pc in ("none", "cnn"):
+    for pc in ("none", "jacobi", "cnn"):
         ks = tl.KrylovStepper(sc.n, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, precond=pc, cnn=cnn, tol=1e-6)
+        ks.sweep(u0)
         _, st = ks.sweep(u0)
         res[f"krylov_{pc}_value"] = P * L / st.seconds
+        res[f"krylov_{pc}_ms"] = st.seconds * 1e3
         res[f"krylov_{pc}_iterations"] = float(np.mean(st.iterations))
         res[f"krylov_{pc}_fallbacks"] = st.fallback_probes
         log(f"3D BiCGSTAB ({pc}): {st.seconds * 1e3:.1f} ms forward sweep, mean iterations "
             f"{res[f'krylov_{pc}_iterations']:.2f}, fallbacks {st.fallback_probes}")
+    res["harder_step"] = harder_step_ablation(dev, objs)
     return res, cnn
+
+
+def harder_step_ablation(dev, objs, dx: float = 0.25, nz: int = 33) -> dict:
+    """The config-3 step at dx = dy = 0.25 (dz = 4): the per-slice operator is worse
+    conditioned (BiCGSTAB without a preconditioner needs ~20 iterations per slice) and
+    the banded LU needs its widest band (half-width 32).  A CNN trained on 20480 pairs
+    at this sampling; forward stepping of the 64 probes through 32 slices of the
+    config-3 object with none / Jacobi / CNN (to 1e-6) and with the banded LU (factor +
+    sweep)."""
+    import numpy as np
+    import torch
+
+    from thick_ptycho_amd import learned as tl
+    from thick_ptycho_amd import workloads
+    from thick_ptycho_amd.forward_model import band_engine as be
+
+    sc = workloads.config3(nz=nz).scans[0]
+    P = sc.probes.shape[0]
+    nx, ny = sc.nx, sc.ny
+    L = nz - 1
+    u0 = torch.from_numpy(sc.probes.reshape(P, -1).astype(np.complex64)).to(dev)
+    pairs = tl.generate_pairs(objs, sc.k, dx, dx, sc.dz, sc.bc, per_slice=64, slices=160, seed=33)
+    cnn = tl.CNNPreconditioner(seed=0)
+    losses = cnn.train(pairs, steps=1500, batch=32, lr=2e-3)
+    del pairs
+    out = {"workload": f"CN stepping {nx}x{ny}x{nz} at dx = dy = {dx}, dz = {sc.dz}, {P} probes, forward, exit planes",
+           "cnn_train_loss": losses[-1]}
+    for pc in ("none", "jacobi", "cnn"):
+        ks = tl.KrylovStepper(sc.n, sc.k, dx, dx, sc.dz, sc.bc, precond=pc, cnn=cnn, tol=1e-6, max_iter=400)
+        ks.sweep(u0)
+        _, st = ks.sweep(u0)
+        out[f"{pc}_ms"] = st.seconds * 1e3
+        out[f"{pc}_iterations"] = float(np.mean(st.iterations))
+        out[f"{pc}_value"] = P * L / st.seconds
+    A, B = be.step_operators_3d(nx, ny, sc.k, dx, dx, sc.dz, sc.bc)
+    C = tl._contribution(sc.n, sc.k, sc.dz, dev)
+    w = be.factor(C, A, B, "forward").w
+    ex = torch.empty_like(u0)
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fac = be.factor(C, A, B, "forward", w=w)
+        be.sweep([be.BandScanSpec(fac, u0, exit=ex)])
+        torch.cuda.synchronize()
+    t_band = time.perf_counter() - t0
+    out.update({"band_halfwidth": w, "band_ms": t_band * 1e3, "band_value": P * L / t_band})
+    log(f"3D step at dx = {dx}: BiCGSTAB none {out['none_iterations']:.2f} its {out['none_ms']:.1f} ms, jacobi "
+        f"{out['jacobi_iterations']:.2f} its {out['jacobi_ms']:.1f} ms, CNN {out['cnn_iterations']:.2f} its "
+        f"{out['cnn_ms']:.1f} ms; banded LU (w = {w}) {out['band_ms']:.1f} ms")
+    return out
 
 
 RECON_FIXTURE = os.path.join(ROOT, "tests", "golden", "recon3d_256x256x128_fp64.npz")

@@ -127,19 +127,48 @@ def test_two_scans_one_launch(gpu, rng):
         assert rel_l2(sp_.exit.cpu().numpy(), ref) < TOL
 
 
-def test_config3_parity(gpu):
-    """Config 3 forward: 128 x 128 x 256, 64 probes, exit waves against the FP64 oracle."""
-    wl = workloads.config3()
-    sc = wl.scans[0]
+@pytest.fixture(scope="module")
+def config3_reference():
+    """Config 3 (128 x 128 x 256, 64 probes) and its FP64 oracle exit waves (SuperLU per slice)."""
+    sc = workloads.config3().scans[0]
     P = sc.probes.shape[0]
     probes = sc.probes.reshape(P, -1)
+    g = pwe.Grid(dimension=3, nx=sc.nx, ny=sc.ny, nz=sc.nz, dx=sc.dx, dy=sc.dy, dz=sc.dz, k=sc.k)
+    ref = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), probes, keep="exit")
+    return sc, probes, ref
+
+
+def test_config3_parity(gpu, config3_reference):
+    """Config 3 forward: 128 x 128 x 256, 64 probes, exit waves against the FP64 oracle."""
+    sc, probes, ref = config3_reference
+    P = probes.shape[0]
     ss = SimulationSpace(sc.nx, sc.nz, sc.dx, sc.dz, wavelength=2 * np.pi / sc.k, ny=sc.ny, dy=sc.dy, num_probes=P)
     s = PWEIterativeLUSolver(ss, probes.reshape(1, P, sc.nx, sc.ny), bc_type=sc.bc)
     got = s.solve_device(n=sc.n, keep="exit")[0, 0].cpu().numpy()
-    g = pwe.Grid(dimension=3, nx=sc.nx, ny=sc.ny, nz=sc.nz, dx=sc.dx, dy=sc.dy, dz=sc.dz, k=sc.k)
-    ref = pwe.solve_batched(g, pwe.build_lu_cache(g, sc.bc, sc.n, "forward"), probes, keep="exit")
     err = rel_l2(got, ref)
     print(f"config 3 exit-wave rel-L2 {err:.2e}, band half-width {s.projection_cache[0].modes['forward'].factors.w}")
+    assert err < TOL
+
+
+def test_config3_learned_krylov_parity(gpu, config3_reference):
+    """Config 3 with the learned form: CN stepping by BiCGSTAB to 1e-6 preconditioned by the
+    CNN (trained on banded-LU residual/correction pairs of two held-out objects), exit
+    waves of all 64 probes after 255 slices against the FP64 oracle (<= 1e-4), with no
+    banded-LU fallback."""
+    from thick_ptycho_amd import learned as tl
+
+    sc, probes, ref = config3_reference
+    P = probes.shape[0]
+    objs = [workloads.ellipsoid_object_3d(sc.nx, sc.ny, 64, 32, np.random.default_rng(s)) for s in (31, 32)]
+    pairs = tl.generate_pairs(objs, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, per_slice=16, slices=32, seed=33)
+    cnn = tl.CNNPreconditioner(seed=0)
+    losses = cnn.train(pairs, steps=400, batch=32, lr=2e-3)
+    ks = tl.KrylovStepper(sc.n, sc.k, sc.dx, sc.dy, sc.dz, sc.bc, precond="cnn", cnn=cnn, tol=1e-6)
+    ex, st = ks.sweep(torch.tensor(probes.astype(np.complex64), device="cuda"))
+    err = rel_l2(ex.cpu().numpy(), ref)
+    print(f"config 3 CNN-preconditioned BiCGSTAB: exit rel-L2 {err:.2e}, mean iterations "
+          f"{np.mean(st.iterations):.2f}, fallbacks {st.fallback_probes}, training loss {losses[-1]:.2e}")
+    assert st.fallback_probes == 0
     assert err < TOL
 
 

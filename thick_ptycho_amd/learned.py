@@ -131,10 +131,15 @@ class PairSet:
 
 
 def generate_pairs(objects, k: float, dx: float, dy: float, dz: float, bc: str, per_slice: int = 8,
-                   slices: Optional[int] = None, seed: int = 31, device=None) -> PairSet:
-    """Residual/correction pairs from the banded LU solver: for each object (nx, ny, nz)
+                   slices: Optional[int] = None, seed: int = 31, device=None, solver: str = "band",
+                   tol: float = 1e-6) -> PairSet:
+    """Residual/correction pairs from the solver itself: for each object (nx, ny, nz)
     and each of ``slices`` slices (evenly spaced), ``per_slice`` random complex planes r
-    and e = A_s^{-1} r (one band step: u_1 = 2 A^{-1} r - r, e = (u_1 + r)/2)."""
+    and e = A_s^{-1} r.  ``solver="band"``: one banded-LU step (u_1 = 2 A^{-1} r - r,
+    e = (u_1 + r)/2); ``solver="krylov"``: unpreconditioned BiCGSTAB to ``tol``, for
+    steps whose inverse blocks decay too slowly for the band (e.g. dx = 0.125 at dz = 4)."""
+    if solver not in ("band", "krylov"):
+        raise ValueError("solver must be 'band' or 'krylov'")
     dev = _dev(device)
     rng = np.random.default_rng(seed)
     rs, es = [], []
@@ -142,19 +147,28 @@ def generate_pairs(objects, k: float, dx: float, dy: float, dz: float, bc: str, 
     for n in objects:
         nx, ny, nz = n.shape
         N = nx * ny
-        A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
-        C = _contribution(n, k, dz, dev)
-        fac = be.factor(C, A, B, "forward")
         L = nz - 1
         picks = np.linspace(0, L - 1, slices or L).round().astype(int)
+        if solver == "band":
+            A, B = be.step_operators_3d(nx, ny, k, dx, dy, dz, bc)
+            C = _contribution(n, k, dz, dev)
+            fac = be.factor(C, A, B, "forward")
+        else:
+            ks = KrylovStepper(n, k, dx, dy, dz, bc, precond="none", tol=tol, max_iter=2000, device=dev)
         for s0 in picks:
-            one = _slice_factors(fac, int(s0))
             r = rng.standard_normal((per_slice, N)) + 1j * rng.standard_normal((per_slice, N))
             r = torch.tensor(r.astype(np.complex64), device=dev)
-            ex = torch.empty_like(r)
-            be.sweep([be.BandScanSpec(one, r, exit=ex)])
+            if solver == "band":
+                ex = torch.empty_like(r)
+                be.sweep([be.BandScanSpec(_slice_factors(fac, int(s0)), r, exit=ex)])
+                e = 0.5 * (ex + r)
+            else:
+                e = torch.zeros_like(r)
+                done = ks.solve_slice(int(s0) + 1, r, e).cpu().numpy()
+                if (done <= 0).any():
+                    raise RuntimeError(f"BiCGSTAB did not reach {tol:g} on slice {int(s0) + 1} for training pairs")
             rs.append(r)
-            es.append(0.5 * (ex + r))
+            es.append(e)
     return PairSet(torch.cat(rs), torch.cat(es), nx, ny)
 
 
