@@ -270,3 +270,30 @@ def test_config4_gradient_at_the_fp64_floor(gpu):
           f"exit plane moved by 1e-16 of its max: {d_floor:.2e}")
     assert d_gpu < 5e-4
     assert d_floor > 1e-4
+
+
+@pytest.mark.parametrize("graph_iters", [4, 0])
+def test_krylov_iteration_graph_equals_eager(gpu, graph_iters):
+    """The learned-Krylov reconstruction iteration captured into a HIP graph (fixed
+    BiCGSTAB iterations per slice, convergence flags on the device, the CNN skipping
+    converged probes) equals the eager iteration; with 0 captured iterations every
+    replay misses the tolerance, is undone and recomputed eagerly with the banded-LU
+    fallback, and still equals the eager run."""
+    from thick_ptycho_amd import learned
+
+    nx, ny, nz, P = 64, 64, 8, 4
+    g, n_true, n0, probes, I = _problem(nx, ny, nz, P, seed=9)
+    cnn = learned.CNNPreconditioner(seed=0)
+    kw = dict(forward="krylov", cnn=cnn, krylov_tol=1e-7, krylov_graph_iters=graph_iters)
+    a = recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet", **kw)
+    b = recon3d.Reconstruction3D(np.sqrt(I), probes, n0, K, 2.0, 2.0, 4.0, "dirichlet", **kw)
+    alpha = 0.5 / b.lipschitz()
+    hb = b.run(3, alpha=alpha, graph=False)
+    ha = a.run(3, alpha=alpha, graph=True)
+    d = rel_l2(a.object() - n0, b.object() - n0)
+    print(f"graph iterations {graph_iters}: redone {a.krylov_redone}, loss {ha['loss']} vs {hb['loss']}, "
+          f"update rel-L2 {d:.2e}")
+    assert a.graph is not None
+    assert a.krylov_redone == (3 if graph_iters == 0 else 0)
+    assert np.allclose(ha["loss"], hb["loss"], rtol=1e-5)
+    assert d < 1e-4

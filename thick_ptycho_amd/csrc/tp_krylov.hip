@@ -26,7 +26,7 @@ namespace tp {
 
 // defined in tp_learn.hip
 int cnn_apply_internal(const void* wb, const float2* in, float2* out, int P, int nx, int ny, void* work,
-                       hipStream_t st);
+                       hipStream_t st, const int* skip);
 size_t cnn_work_bytes_internal(int P, int nx, int ny);
 
 struct KOp {
@@ -382,14 +382,14 @@ extern "C" int tp_bicgstab_solve(const void* d_diag, const void* d_tri32, const 
   for (int k = 0; k < max_iter; ++k) {
     if (precond == 1) hipLaunchKernelGGL(k_jacobi, grid, blk, 0, st, s, op.diag, (const float2*)w.p, ph);
     if (precond == 2) {
-      rc = cnn_apply_internal(d_cnn_w, w.p, ph, P, nx, ny, w.cnn, st);
+      rc = cnn_apply_internal(d_cnn_w, w.p, ph, P, nx, ny, w.cnn, st, w.done);
       if (rc) break;
     }
     hipLaunchKernelGGL(k_spmv_v, grid, blk, 0, st, op, s, k, (const float2*)ph, w.v, (const float2*)w.rh);
     hipLaunchKernelGGL(k_s, grid, blk, 0, st, s, k, (const float2*)w.r, (const float2*)w.v, w.s);
     if (precond == 1) hipLaunchKernelGGL(k_jacobi, grid, blk, 0, st, s, op.diag, (const float2*)w.s, sh);
     if (precond == 2) {
-      rc = cnn_apply_internal(d_cnn_w, w.s, sh, P, nx, ny, w.cnn, st);
+      rc = cnn_apply_internal(d_cnn_w, w.s, sh, P, nx, ny, w.cnn, st, w.done);
       if (rc) break;
     }
     hipLaunchKernelGGL(k_spmv_t, grid, blk, 0, st, op, s, k, (const float2*)sh, w.t, (const float2*)w.s);

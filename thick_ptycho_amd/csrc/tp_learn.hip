@@ -57,6 +57,7 @@ struct ConvArgs {
   bf16* out;
   float* outc;
   const float* scale;
+  const int* skip;  // optional per-image flags: images with a nonzero flag are not computed
   int P, H, W, tiles_x, tiles_y;
 };
 
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a) {
   const bf16 bz = (bf16)0.f;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int img = tile / per_img;
+    if (a.skip && a.skip[img] != 0) continue;  // uniform over the workgroup
     const int rem = tile - img * per_img;
     const int y0 = (rem / a.tiles_x) * TH, x0 = (rem % a.tiles_x) * TW;
     __syncthreads();
@@ -271,8 +273,9 @@ __device__ __forceinline__ void wg_atomic_add(double s, double* dst) {
 
 // complex64 planes -> BF16 NHWC (re, im) divided by the plane's RMS (kept in scale)
 __global__ __launch_bounds__(256) void cnn_norm_kernel(const float2* __restrict__ v, int64_t N,
-                                                       double* __restrict__ nrm2) {
+                                                       double* __restrict__ nrm2, const int* __restrict__ skip) {
   const int p = blockIdx.y;
+  if (skip && skip[p] != 0) return;  // uniform over the workgroup
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     const float2 x = v[(size_t)p * N + i];
@@ -282,8 +285,9 @@ __global__ __launch_bounds__(256) void cnn_norm_kernel(const float2* __restrict_
 }
 
 __global__ void cnn_pack_kernel(const float2* __restrict__ v, int64_t N, const double* __restrict__ nrm2,
-                                bf16* __restrict__ x0, float* __restrict__ scale) {
+                                bf16* __restrict__ x0, float* __restrict__ scale, const int* __restrict__ skip) {
   const int p = blockIdx.y;
+  if (skip && skip[p] != 0) return;
   const double ms = nrm2[p] / (double)N;
   const float s = ms > 0.0 ? (float)sqrt(ms) : 1.f;
   const float inv = 1.f / s;
@@ -436,8 +440,9 @@ static size_t cnn_work_layout(int P, int64_t N, int train, CnnWork* w, char* bas
 
 template <int CIN, int COUT, bool FINAL>
 static void launch_conv(const bf16* in, const bf16* w, bf16* out, float* outc, const float* scale, int P, int nx,
-                        int ny, hipStream_t st) {
+                        int ny, hipStream_t st, const int* skip = nullptr) {
   ConvArgs a;
+  a.skip = skip;
   a.in = in;
   a.w = w;
   a.out = out;
@@ -481,17 +486,18 @@ static void zero_words(void* p, size_t bytes, hipStream_t st) {
 }
 
 // forward pass through the four layers; x3 holds layer 3's output, the result goes to out (complex64)
+// `skip` (optional, per plane): planes with a nonzero flag are left unwritten
 static void cnn_forward(const bf16* wb, const float2* in, float2* out, int P, int nx, int ny, const CnnWork& w,
-                        hipStream_t st) {
+                        hipStream_t st, const int* skip = nullptr) {
   const int64_t N = (int64_t)nx * ny;
   zero_words(w.nrm2, (size_t)P * sizeof(double), st);
   const dim3 g(N / 256 + 1 < 16 ? (unsigned)(N / 256 + 1) : 16u, (unsigned)P);
-  hipLaunchKernelGGL(cnn_norm_kernel, g, dim3(256), 0, st, in, N, w.nrm2);
-  hipLaunchKernelGGL(cnn_pack_kernel, g, dim3(256), 0, st, in, N, w.nrm2, w.x0, w.scale);
-  launch_conv<2, 32, false>(w.x0, wb + OFF_W1, w.x1, nullptr, nullptr, P, nx, ny, st);
-  launch_conv<32, 32, false>(w.x1, wb + OFF_W2, w.x2, nullptr, nullptr, P, nx, ny, st);
-  launch_conv<32, 32, false>(w.x2, wb + OFF_W3, w.x3, nullptr, nullptr, P, nx, ny, st);
-  launch_conv<32, 2, true>(w.x3, wb + OFF_W4, nullptr, (float*)out, w.scale, P, nx, ny, st);
+  hipLaunchKernelGGL(cnn_norm_kernel, g, dim3(256), 0, st, in, N, w.nrm2, skip);
+  hipLaunchKernelGGL(cnn_pack_kernel, g, dim3(256), 0, st, in, N, w.nrm2, w.x0, w.scale, skip);
+  launch_conv<2, 32, false>(w.x0, wb + OFF_W1, w.x1, nullptr, nullptr, P, nx, ny, st, skip);
+  launch_conv<32, 32, false>(w.x1, wb + OFF_W2, w.x2, nullptr, nullptr, P, nx, ny, st, skip);
+  launch_conv<32, 32, false>(w.x2, wb + OFF_W3, w.x3, nullptr, nullptr, P, nx, ny, st, skip);
+  launch_conv<32, 2, true>(w.x3, wb + OFF_W4, nullptr, (float*)out, w.scale, P, nx, ny, st, skip);
 }
 
 static int check_cnn_dims(int P, int nx, int ny) {
@@ -504,10 +510,10 @@ static int check_cnn_dims(int P, int nx, int ny) {
 
 // entry points for the Krylov solver in the same library
 int cnn_apply_internal(const void* wb, const float2* in, float2* out, int P, int nx, int ny, void* work,
-                       hipStream_t st) {
+                       hipStream_t st, const int* skip) {
   CnnWork w;
   cnn_work_layout(P, (int64_t)nx * ny, 0, &w, (char*)work);
-  cnn_forward((const bf16*)wb, in, out, P, nx, ny, w, st);
+  cnn_forward((const bf16*)wb, in, out, P, nx, ny, w, st, skip);
   TP_CHECK_HIP(hipGetLastError());
   return TP_OK;
 }

@@ -219,6 +219,7 @@ class KrylovStepper:
             raise ValueError("precond='cnn' needs a CNNPreconditioner")
         self.precond, self.cnn, self.tol, self.max_iter = precond, cnn, tol, max_iter
         self._band = None
+        self._bufs = {}
 
     def _band_slice(self, s: int) -> be.BandFactors:
         """Banded factors of step s alone (the fallback), factored on first use."""
@@ -229,20 +230,35 @@ class KrylovStepper:
             self._band[s] = be.factor(col, self.A, self.B, "forward")
         return self._band[s]
 
+    def _buffers(self, P: int, mi: int):
+        """Per-(probe count, iteration cap) buffers, allocated once (graph capture replays them)."""
+        key = (P, mi)
+        if key not in self._bufs:
+            lib = _lib.load()
+            pc = PRECONDITIONERS[self.precond]
+            self._bufs[key] = (
+                torch.empty(self.N, dtype=torch.complex64, device=self.dev),
+                torch.empty(int(lib.tp_bicgstab_workspace_bytes(P, self.nx, self.ny, max(mi, 1), pc)),
+                            dtype=torch.uint8, device=self.dev),
+                torch.zeros(P, dtype=torch.int32, device=self.dev),
+            )
+        return self._bufs[key]
+
     def solve_slice(self, s: int, b: torch.Tensor, x: torch.Tensor, max_iter: Optional[int] = None,
                     check_every: int = 1) -> torch.Tensor:
-        """BiCGSTAB for A_s x = b (x holds the initial guess); returns done flags (int [P])."""
+        """BiCGSTAB for A_s x = b (x holds the initial guess); returns done flags (int [P],
+        the iterations taken, or <= 0 where the probe did not converge), a buffer reused by
+        the next call.  ``check_every`` = 0 never reads the flags on the host: all
+        ``max_iter`` iterations are launched (converged probes skip their work), so the
+        call can be captured into a HIP graph."""
         lib = _lib.load()
         P = b.shape[0]
         mi = self.max_iter if max_iter is None else max_iter
-        diag = torch.empty(self.N, dtype=torch.complex64, device=self.dev)
+        diag, work, done = self._buffers(P, mi)
         sh = _lib.stream_handle()
         _lib.check(lib.tp_krylov_diag(_lib.ptr(self.C), self.nx, self.ny, self.nz - 1, _lib.ptr(self.tri64), -1.0, 0,
                                       s - 1, _lib.ptr(diag), sh), "tp_krylov_diag")
         pc = PRECONDITIONERS[self.precond]
-        work = torch.empty(int(lib.tp_bicgstab_workspace_bytes(P, self.nx, self.ny, max(mi, 1), pc)),
-                           dtype=torch.uint8, device=self.dev)
-        done = torch.zeros(P, dtype=torch.int32, device=self.dev)
         _lib.check(
             lib.tp_bicgstab_solve(_lib.ptr(diag), _lib.ptr(self.tri32), _lib.ptr(self.cpl32), self.nx, self.ny, P,
                                   _lib.ptr(b), _lib.ptr(x), pc, _lib.ptr(self.cnn.wbuf) if pc == 2 else 0, self.tol,
@@ -268,7 +284,7 @@ class KrylovStepper:
         for s in range(1, self.nz):
             x.copy_(u)
             done = self.solve_slice(s, u, x, max_iter)
-            d = done.cpu().numpy()
+            d = done.cpu().numpy().copy()
             bad = np.nonzero(d <= 0)[0]
             good = d[d > 0]
             stats.iterations.append(float(good.mean()) if good.size else float("nan"))
@@ -292,3 +308,29 @@ class KrylovStepper:
         torch.cuda.synchronize()
         stats.seconds = time.perf_counter() - t0
         return (full if full is not None else u), stats
+
+    def sweep_device(self, u0: torch.Tensor, out: torch.Tensor, done_log: torch.Tensor,
+                     max_iter: Optional[int] = None) -> None:
+        """Forward sweep without host synchronisation (capturable into a HIP graph):
+        every slice runs ``max_iter`` BiCGSTAB iterations with the convergence flags kept
+        on the device (converged probes skip their work, the CNN included), the planes go
+        to ``out`` (P, nz, N) complex64 and each slice's flags to ``done_log`` (nz - 1, P)
+        int32.  No banded-LU fallback: the caller reads ``done_log`` afterwards (a probe
+        with a flag <= 0 did not converge on that slice) and recomputes with ``sweep``."""
+        lib = _lib.load()
+        P = u0.shape[0]
+        assert out.shape == (P, self.nz, self.N) and out.dtype == torch.complex64 and out.is_contiguous()
+        assert done_log.shape == (self.nz - 1, P) and done_log.dtype == torch.int32
+        out[:, 0].copy_(u0)
+        x = self._bufs.setdefault(("x", P), torch.empty((P, self.N), dtype=torch.complex64, device=self.dev))
+        for s in range(1, self.nz):
+            u = out[:, s - 1]
+            x.copy_(u)
+            ucont = self._bufs.setdefault(("u", P), torch.empty((P, self.N), dtype=torch.complex64, device=self.dev))
+            ucont.copy_(u)
+            done = self.solve_slice(s, ucont, x, max_iter, check_every=0)
+            done_log[s - 1].copy_(done)
+            nxt = self._bufs.setdefault(("n", P), torch.empty((P, self.N), dtype=torch.complex64, device=self.dev))
+            _lib.check(lib.tp_cn_finish(P * self.N, _lib.ptr(x), _lib.ptr(ucont), _lib.ptr(nxt), 2.0,
+                                        _lib.stream_handle()), "tp_cn_finish")
+            out[:, s].copy_(nxt)

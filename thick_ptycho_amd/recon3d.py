@@ -14,10 +14,13 @@ out of scope here); this loop exercises the same forward / adjoint / gradient
 kernels for two transverse axes.
 
 With ``forward="krylov"`` the forward planes come from the learned-preconditioned
-BiCGSTAB stepper instead (``learned.KrylovStepper`` on the current object, the
-banded LU of a slice only where BiCGSTAB misses the tolerance); the adjoint and
-the gradient stay on the banded LU.  That path synchronises with the host on
-every slice, so its iterations run eagerly (no graph).
+BiCGSTAB stepper instead (``learned.KrylovStepper`` on the current object); the
+adjoint and the gradient stay on the banded LU.  Captured into a HIP graph, each
+slice runs a fixed number of BiCGSTAB iterations (``krylov_graph_iters``) with the
+convergence flags kept on the device; after each replay the flags are read once,
+and an iteration in which a probe missed the tolerance on some slice is undone and
+recomputed eagerly, with the banded LU of that slice for the probes that miss it
+(the fallback).
 """
 
 from __future__ import annotations
@@ -34,7 +37,8 @@ from .forward_model import band_engine as be
 class Reconstruction3D:
     def __init__(self, sqrt_i, probes, n0, k: float, dx: float, dy: float, dz: float, bc: str = "dirichlet",
                  batch: Optional[int] = None, w: Optional[int] = None, probes_per_block: int = 1, device=None,
-                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6, exit_data=None, refine: int = 0):
+                 forward: str = "band", cnn=None, krylov_tol: float = 1e-6, exit_data=None, refine: int = 0,
+                 krylov_graph_iters: int = 4):
         """``sqrt_i`` (P, nx, ny): measured far-field amplitudes (float64 amplitudes are
         kept as float32 high and low parts); or ``sqrt_i=None`` with ``exit_data``
         (P, nx, ny) complex exit waves and the loss sum |u_L - exit_data|^2 (the
@@ -120,6 +124,11 @@ class Reconstruction3D:
             self.krylov = learned.KrylovStepper(n0, k, dx, dy, dz, bc, precond="cnn" if cnn is not None else "none",
                                                 cnn=cnn, tol=krylov_tol, device=dev)
             self.krylov.C = self.C  # the stepper follows the object being reconstructed
+            self.krylov_graph_iters = krylov_graph_iters
+            nbatch = (P + self.batch - 1) // self.batch
+            self.krylov_done = torch.ones((nbatch, self.L, self.batch), dtype=torch.int32, device=dev)
+            self._krylov_device = False  # sweep_device (inside a graph capture) or sweep with fallback
+            self.krylov_redone = 0
 
     def _contribution(self) -> None:
         coef = (self.k / 1j) * (self.n.reshape(self.N, self.nz) - 1)
@@ -142,7 +151,10 @@ class Reconstruction3D:
             b1 = min(b0 + self.batch, self.P)
             nb = b1 - b0
             u, lam = self.u[:nb], self.lam[:nb]
-            if self.krylov is not None:
+            if self.krylov is not None and self._krylov_device:
+                self.krylov.sweep_device(self.probes[b0:b1], u, self.krylov_done[b0 // self.batch, :, :nb],
+                                         max_iter=self.krylov_graph_iters)
+            elif self.krylov is not None:
                 planes, st = self.krylov.sweep(self.probes[b0:b1], keep="all")
                 u.copy_(planes)
                 self.krylov_stats.append(st)
@@ -244,6 +256,7 @@ class Reconstruction3D:
 
     def capture(self) -> None:
         """Capture one iteration (gradient + update) into a HIP graph."""
+        self._krylov_device = self.krylov is not None
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         n_save = self.n.clone()
@@ -255,19 +268,31 @@ class Reconstruction3D:
             self.gradient()
             self._update()
         self.n.copy_(n_save)  # the capture does not execute; keep n as it was
+        self._krylov_device = False
 
     def run(self, iters: int, alpha: Optional[float] = None, graph: bool = True, check_band: bool = True) -> dict:
         """Steepest descent; returns {'loss': [J before each iteration], 'alpha': ...}."""
         if alpha is None:
             alpha = 0.5 / self.lipschitz()
         self.alpha.fill_(alpha)
-        graph = graph and self.krylov is None and self.exit_data is None  # torch reductions stay out of graphs
+        graph = graph and self.exit_data is None  # torch reductions stay out of graphs
         if graph and self.graph is None:
             self.capture()
         hist: List[float] = []
+        n_prev = torch.empty_like(self.n) if (graph and self.krylov is not None) else None
         for _ in range(iters):
             if graph:
+                if n_prev is not None:
+                    n_prev.copy_(self.n)
+                    self.krylov_done.fill_(1)
                 self.graph.replay()
+                if n_prev is not None and int(self.krylov_done.min().item()) <= 0:
+                    # a probe missed the tolerance within the captured iterations on some
+                    # slice: undo the step and redo it eagerly with the banded-LU fallback
+                    self.n.copy_(n_prev)
+                    self.krylov_redone += 1
+                    self.gradient()
+                    self._update()
             else:
                 self.gradient()
                 self._update()
