@@ -174,6 +174,70 @@ __device__ __forceinline__ void fft_wave(float2* x, float2* a, float2* b, const 
   }
 }
 
+// The 256-point wave transform (fft_wave<4>) with its twiddles in registers.
+// Every call reads the same powers of W_256 (per lane: three for each of the
+// two LDS stages and three for the last), so a sweep loads them once
+// (tw256_load) instead of once per transform from the LDS table, whose lookups
+// at lane strides of 2 and 3 bank-conflict.  Forward powers are kept; the
+// inverse uses their conjugates.
+struct Tw256 {
+  float2 s1[3], s2[3], s3[3];
+};
+
+__device__ __forceinline__ void tw256_load(Tw256& t, const float2* tw, int lane) {
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    t.s1[r - 1] = tw[(lane & 3) * r * 16];  // stage NS = 4:  W_256^{(j mod 4) r 16}
+    t.s2[r - 1] = tw[(lane & 15) * r * 4];  // stage NS = 16: W_256^{(j mod 16) r 4}
+    t.s3[r - 1] = tw[lane * r];             // last stage:    W_256^{j r}
+  }
+}
+
+template <int SIGN>
+__device__ __forceinline__ float2 tw_sign(float2 w) {
+  return SIGN > 0 ? cconj(w) : w;
+}
+
+// fft_wave<4, SIGN, true> with the twiddles of `t`: the same stages, the same
+// buffers and paddings (b takes the wide padding), natural order in and out.
+template <int SIGN>
+__device__ __forceinline__ void fft_wave4_rt(float2* x, float2* a, float2* b, const Tw256& t, int lane) {
+  stage_from_reg<4, SIGN>(x, a, lane);
+  const int j = lane;
+  {  // radix 4, NS = 4: a (lp) -> b (lp4)
+    float2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = a[lp(j + r * 64)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s1[r - 1]));
+    dft_reg<4, SIGN>(v);
+    const int base = (j >> 2) * 16 + (j & 3);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) b[lp4(base + r * 4)] = v[r];
+  }
+  {  // radix 4, NS = 16: b (lp4) -> a (lp)
+    float2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = b[lp4(j + r * 64)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s2[r - 1]));
+    dft_reg<4, SIGN>(v);
+    const int base = (j >> 4) * 64 + (j & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[lp(base + r * 16)] = v[r];
+  }
+  {  // last stage: a (lp) -> registers, natural order
+    float2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = a[lp(j + r * 64)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s3[r - 1]));
+    dft_reg<4, SIGN>(v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = v[r];
+  }
+}
+
 // Twiddle table W_N^e, e < N, built per workgroup in FP64.
 __device__ __forceinline__ void build_twiddles(float2* tw, int n) {
   for (int e = threadIdx.x; e < n; e += blockDim.x) {

@@ -133,10 +133,63 @@ __device__ __forceinline__ void ms_cross(float2* x, float2* ta, float2* tb, cons
   for (int m = 0; m < RM; ++m) x[m] = tb[lp(w * M + lane + 64 * m)];
 }
 
+// ms_cross's twiddles, per lane and fixed for a sweep, held in registers:
+// fwd[t][k] = W_n^{n2 k} for the forward exchange (n2 = w M/S + lane + 64 t),
+// inv[m] = W_n^{(lane + 64 m) w} (conjugated when used) for the inverse.
+template <int S, int RM>
+struct CrossTw {
+  float2 fwd[RM / S][S];
+  float2 inv[RM];
+};
+
+template <int S, int RM>
+__device__ __forceinline__ void cross_tw_load(CrossTw<S, RM>& c, const float2* twN, int w, int lane) {
+  constexpr int M = 64 * RM, N = S * M, QL = M / S / 64;
+#pragma unroll
+  for (int t = 0; t < QL; ++t) {
+    const int n2 = w * (M / S) + lane + 64 * t;
+#pragma unroll
+    for (int k = 1; k < S; ++k) c.fwd[t][k] = twN[(n2 * k) & (N - 1)];
+  }
+#pragma unroll
+  for (int m = 0; m < RM; ++m) c.inv[m] = twN[((lane + 64 * m) * w) & (N - 1)];
+}
+
+// ms_cross with the twiddles of `c` (same exchange, buffers and order)
+template <int S, int RM, int SIGN>
+__device__ __forceinline__ void ms_cross_rt(float2* x, float2* ta, float2* tb, const CrossTw<S, RM>& c, int w,
+                                            int lane) {
+  constexpr int M = 64 * RM, QL = M / S / 64;
+  if (SIGN > 0) {
+#pragma unroll
+    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], cconj(c.inv[m]));
+  }
+#pragma unroll
+  for (int m = 0; m < RM; ++m) ta[lp(w * M + lane + 64 * m)] = x[m];
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < QL; ++t) {
+    const int n2 = w * (M / S) + lane + 64 * t;
+    float2 v[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) v[r] = ta[lp(r * M + n2)];
+    dft_reg<S, SIGN>(v);
+    if (SIGN < 0) {
+#pragma unroll
+      for (int k = 1; k < S; ++k) v[k] = cmul(v[k], c.fwd[t][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) tb[lp(k * M + n2)] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < RM; ++m) x[m] = tb[lp(w * M + lane + 64 * m)];
+}
+
 template <int R, int S>
 __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
   constexpr int N = 64 * R, M = N / S, RM = R / S;
-  static_assert(ms_pow2<RM>() && M % (64 * S) == 0, "split form needs a power-of-two wave FFT");
+  static_assert(RM == 4 && M % (64 * S) == 0, "split form: 256-point wave transforms");
   __shared__ float2 twN[N];
   __shared__ float2 twM[M];
   __shared__ float2 ta[N + N / 16], tb[N + N / 16];
@@ -150,6 +203,10 @@ __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
   build_twiddles(twN, N);
   build_twiddles(twM, M);
   __syncthreads();
+  CrossTw<S, RM> ctw;
+  cross_tw_load<S, RM>(ctw, twN, w, lane);
+  Tw256 wtw;
+  tw256_load(wtw, twM, lane);
   float2 H[RM], T[RM], Tn[RM], x[RM];
   const float2* Hg = (const float2*)d.H;
   const float2* Tg = (const float2*)d.T + w * M;
@@ -173,12 +230,12 @@ __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
     }
 #pragma unroll
     for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], T[m]);
-    ms_cross<S, RM, -1>(x, ta, tb, twN, w, lane);
-    fft_wave<RM, -1>(x, buf[w][0], buf[w][1], twM, lane);
+    ms_cross_rt<S, RM, -1>(x, ta, tb, ctw, w, lane);
+    fft_wave4_rt<-1>(x, buf[w][0], buf[w][1], wtw, lane);
 #pragma unroll
     for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], H[m]);
-    fft_wave<RM, +1>(x, buf[w][0], buf[w][1], twM, lane);
-    ms_cross<S, RM, +1>(x, ta, tb, twN, w, lane);
+    fft_wave4_rt<+1>(x, buf[w][0], buf[w][1], wtw, lane);
+    ms_cross_rt<S, RM, +1>(x, ta, tb, ctw, w, lane);
     if (outp) {
       float2* o = outp + (int64_t)(z + 1) * d.out_zstride;
 #pragma unroll
