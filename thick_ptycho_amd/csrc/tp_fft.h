@@ -198,38 +198,47 @@ __device__ __forceinline__ float2 tw_sign(float2 w) {
   return SIGN > 0 ? cconj(w) : w;
 }
 
-// fft_wave<4, SIGN, true> with the twiddles of `t`: the same stages, the same
-// buffers and paddings (b takes the wide padding), natural order in and out.
+// Paddings of the two buffers of fft_wave4_rt: one slot per 32 values in a, four
+// in b (gfx950's LDS serves 64 banks of 4 bytes; these keep each half-wavefront's
+// 8-byte accesses of every stage on distinct banks, where the 16-value paddings
+// of fft_wave, sized for 32 banks, leave two-way conflicts).
+__device__ __forceinline__ constexpr int lpa(int i) { return i + (i >> 5); }
+__device__ __forceinline__ constexpr int lpb(int i) { return i + ((i >> 5) << 2); }
+
+// fft_wave<4, SIGN, true> with the twiddles of `t`: the same stages and
+// buffers (at least ms_bufn<4>() points each), natural order in and out.
 template <int SIGN>
 __device__ __forceinline__ void fft_wave4_rt(float2* x, float2* a, float2* b, const Tw256& t, int lane) {
-  stage_from_reg<4, SIGN>(x, a, lane);
   const int j = lane;
-  {  // radix 4, NS = 4: a (lp) -> b (lp4)
+  dft_reg<4, SIGN>(x);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[lpa(j * 4 + r)] = x[r];
+  {  // radix 4, NS = 4: a -> b
     float2 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = a[lp(j + r * 64)];
+    for (int r = 0; r < 4; ++r) v[r] = a[lpa(j + r * 64)];
 #pragma unroll
     for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s1[r - 1]));
     dft_reg<4, SIGN>(v);
     const int base = (j >> 2) * 16 + (j & 3);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) b[lp4(base + r * 4)] = v[r];
+    for (int r = 0; r < 4; ++r) b[lpb(base + r * 4)] = v[r];
   }
-  {  // radix 4, NS = 16: b (lp4) -> a (lp)
+  {  // radix 4, NS = 16: b -> a
     float2 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = b[lp4(j + r * 64)];
+    for (int r = 0; r < 4; ++r) v[r] = b[lpb(j + r * 64)];
 #pragma unroll
     for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s2[r - 1]));
     dft_reg<4, SIGN>(v);
     const int base = (j >> 4) * 64 + (j & 15);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[lp(base + r * 16)] = v[r];
+    for (int r = 0; r < 4; ++r) a[lpa(base + r * 16)] = v[r];
   }
-  {  // last stage: a (lp) -> registers, natural order
+  {  // last stage: a -> registers, natural order
     float2 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = a[lp(j + r * 64)];
+    for (int r = 0; r < 4; ++r) v[r] = a[lpa(j + r * 64)];
 #pragma unroll
     for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], tw_sign<SIGN>(t.s3[r - 1]));
     dft_reg<4, SIGN>(v);

@@ -155,7 +155,8 @@ __device__ __forceinline__ void cross_tw_load(CrossTw<S, RM>& c, const float2* t
   for (int m = 0; m < RM; ++m) c.inv[m] = twN[((lane + 64 * m) * w) & (N - 1)];
 }
 
-// ms_cross with the twiddles of `c` (same exchange, buffers and order)
+// ms_cross with the twiddles of `c` (same exchange and order).  Every access of
+// the exchange is to consecutive values, so the buffers take no padding.
 template <int S, int RM, int SIGN>
 __device__ __forceinline__ void ms_cross_rt(float2* x, float2* ta, float2* tb, const CrossTw<S, RM>& c, int w,
                                             int lane) {
@@ -165,25 +166,25 @@ __device__ __forceinline__ void ms_cross_rt(float2* x, float2* ta, float2* tb, c
     for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], cconj(c.inv[m]));
   }
 #pragma unroll
-  for (int m = 0; m < RM; ++m) ta[lp(w * M + lane + 64 * m)] = x[m];
+  for (int m = 0; m < RM; ++m) ta[w * M + lane + 64 * m] = x[m];
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < QL; ++t) {
     const int n2 = w * (M / S) + lane + 64 * t;
     float2 v[S];
 #pragma unroll
-    for (int r = 0; r < S; ++r) v[r] = ta[lp(r * M + n2)];
+    for (int r = 0; r < S; ++r) v[r] = ta[r * M + n2];
     dft_reg<S, SIGN>(v);
     if (SIGN < 0) {
 #pragma unroll
       for (int k = 1; k < S; ++k) v[k] = cmul(v[k], c.fwd[t][k]);
     }
 #pragma unroll
-    for (int k = 0; k < S; ++k) tb[lp(k * M + n2)] = v[k];
+    for (int k = 0; k < S; ++k) tb[k * M + n2] = v[k];
   }
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < RM; ++m) x[m] = tb[lp(w * M + lane + 64 * m)];
+  for (int m = 0; m < RM; ++m) x[m] = tb[w * M + lane + 64 * m];
 }
 
 template <int R, int S>
