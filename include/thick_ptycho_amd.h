@@ -81,7 +81,7 @@
 extern "C" {
 #endif
 
-#define TP_ABI_VERSION 2
+#define TP_ABI_VERSION 3
 
 #define TP_OK 0
 #define TP_ERR_INVALID (-1)     /* bad argument or unsupported size      */
@@ -125,7 +125,14 @@ size_t tp_tri_coef_bytes(int n, int steps, int chunk);
  *      (sub[0] and sup[n-1] are ignored).
  * One wavefront factors one step: pivots in FP64 (no pivot search: the CN
  * operators are strictly diagonally dominant), products stored in complex64.
- * d_status (optional, caller-zeroed int) is set to 1 if a pivot is exactly 0. */
+ * d_status (optional, caller-zeroed int[2]): [0] is set to 1 if a pivot is
+ * exactly 0; [1] is raised (atomic max) to the coupling reach of the factors:
+ * the largest number of rows of a chunk, counted from its first row forward or
+ * its last row backward, over which the carry from the neighbouring chunk
+ * keeps a factor above 2^-32 (|re| + |im| of prod(-l) resp. prod(q) in FP64).
+ * A reach below `chunk` lets tp_tri_sweep drop the 64-lane carry scans (see
+ * tp_tri_scan.reach).  Factor sets that share one d_status get the reach of
+ * their union. */
 int tp_tri_factor(const void* d_C, int n, int ncol, const void* d_base,
                   double lhs_sign, double rhs_sign, int conj_c,
                   int lhs_col0, int rhs_col0, int col_step, int steps,
@@ -187,6 +194,15 @@ typedef struct tp_tri_scan {
    * as many rows per lane beside the wide ones of the same persistent launch.
    * The plain one-step form takes 0 only. */
   int chunk;
+  /* Optional (NULL = full carries): d_status + 1 of the factorisation(s) that
+   * wrote `coef` (all of them, for per-probe factors).  When *reach is below
+   * the scan's chunk (8 rows or more), tp_tri_sweep takes the near-carry step:
+   * the carry into a chunk is the neighbouring chunk's local end value, applied
+   * to the first / last *reach rows only, with no 64-lane scan; the terms it
+   * leaves out are below 2^-32 of the carried values.  Read on the device when
+   * the sweep starts, so a sweep queued behind its factorisation needs no host
+   * synchronisation.  tp_tri_step and tp_tri_sweep_wide ignore it. */
+  const int* reach;
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe
@@ -214,7 +230,8 @@ int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int s, void* st
  * (n <= 2048) do not hold. */
 size_t tp_tri_coef_bytes_wide(int n, int steps, int chunk, int waves);
 
-/* tp_tri_factor for the wide layout: one workgroup of 64 * waves lanes per step. */
+/* tp_tri_factor for the wide layout: one workgroup of 64 * waves lanes per step
+ * (d_status[1] is raised to 64: the wide sweep carries across every chunk). */
 int tp_tri_factor_wide(const void* d_C, int n, int ncol, const void* d_base,
                        double lhs_sign, double rhs_sign, int conj_c,
                        int lhs_col0, int rhs_col0, int col_step, int steps,

@@ -1,0 +1,127 @@
+"""Near-carry form of the persistent tridiagonal sweep.
+
+The factor kernel records the coupling reach of its factors (the rows over
+which a carry from a neighbouring 64-lane chunk keeps a factor above 2^-32);
+below the chunk, the sweep replaces the two 64-lane affine scans by the
+neighbouring chunk's local end value and applies it to `reach` rows only.
+
+Checks: the reach equals the one of a sequential FP64 Thomas factorisation of
+the same per-slice operators; near and full carries give the same planes to
+FP32 rounding (<= 1e-6 relative L2); both meet the FP64 oracle at 1e-4 where
+the coupling is weak (config-1 parameters, dx = 2, dz = 4), intermediate
+(dx = 1) and strong enough that the reach covers a whole chunk (dx = 0.25,
+which takes the full scans).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pwe
+from tests.cases import random_object, rel_l2
+from thick_ptycho_amd.forward_model import operators, tri_engine
+
+pytestmark = pytest.mark.gpu
+TAU = 2.0 ** -32
+
+
+def _fp64_reach(g, bc, n, chunk):
+    """Largest reach over the slices of the forward mode, from a sequential
+    FP64 Thomas factorisation of A0 - diag(C[:, s-1]) split into 64 chunks."""
+    A0, _ = pwe.step_operators_1d(g, bc)
+    C = pwe.object_contribution(g, n)
+    nx, L = C.shape
+    sub0, dia0, sup0 = A0.diagonal(-1), A0.diagonal(0), A0.diagonal(1)
+    best = 0
+    for s in range(L):
+        d = dia0 - C[:, s]
+        piv = np.empty(nx, complex)
+        lmul = np.zeros(nx, complex)
+        q = np.zeros(nx, complex)
+        piv[0] = d[0]
+        for i in range(1, nx):
+            lmul[i] = sub0[i - 1] / piv[i - 1]
+            piv[i] = d[i] - lmul[i] * sup0[i - 1]
+        q[:-1] = -sup0 / piv[:-1]
+        for lane in range(64):
+            r0 = lane * chunk
+            rows = range(r0, min(r0 + chunk, nx))
+            gf = 1.0 + 0j
+            for m, i in enumerate(rows):
+                gf *= -lmul[i]
+                if abs(gf.real) + abs(gf.imag) > TAU:
+                    best = max(best, m + 1)
+            gb = 1.0 + 0j
+            for m in range(chunk - 1, -1, -1):
+                i = r0 + m
+                gb *= q[i] if i < nx else 0.0
+                if abs(gb.real) + abs(gb.imag) > TAU:
+                    best = max(best, chunk - m)
+    return best
+
+
+def _case(nx, nz, dx, P, rng):
+    k = 2 * np.pi
+    g = pwe.Grid(dimension=2, nx=nx, nz=nz, dx=dx, dz=4.0, k=k)
+    n = random_object(nx, nz, rng, amp=1e-3)
+    probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+    return g, n, probes
+
+
+def _sweep(g, bc, n, probes, mode, chunk, near):
+    A, B = operators.step_operators_1d(g.nx, g.k, g.dx, g.dz, bc)
+    C = torch.from_numpy(pwe.object_contribution(g, n)).to("cuda")
+    fac = tri_engine.factor(C, A, B, mode, chunk=chunk)
+    u0 = torch.from_numpy(probes).to("cuda", torch.complex64)
+    out = torch.empty((u0.shape[0], g.nz, g.nx), dtype=torch.complex64, device="cuda")
+    tri_engine.sweep([tri_engine.TriScanSpec(fac, u0, out=out, near=near)])
+    return fac, out
+
+
+@pytest.mark.parametrize("dx,expect", [(2.0, "near"), (1.0, "near"), (0.25, "full")])
+def test_reach_matches_fp64_thomas(gpu, rng, dx, expect):
+    print(f"case reach dx={dx}", flush=True)
+    g, n, probes = _case(1024, 9, dx, 2, rng)
+    fac, _ = _sweep(g, "impedance", n, probes, "forward", 16, True)
+    got = int(fac.status[1].item())
+    ref = _fp64_reach(g, "impedance", n, 16)
+    assert abs(got - ref) <= 1, (got, ref)
+    assert (got < 16) == (expect == "near"), got
+
+
+@pytest.mark.parametrize("dx", [2.0, 1.0, 0.25])
+@pytest.mark.parametrize("mode", ["forward", "adjoint", "exact_adjoint"])
+@pytest.mark.parametrize("nx,chunk", [(1024, 16), (700, 16), (512, 8), (2048, 32)])
+def test_near_and_full_carries_agree(gpu, rng, dx, mode, nx, chunk):
+    print(f"case near/full dx={dx} mode={mode} nx={nx} chunk={chunk}", flush=True)
+    g, n, probes = _case(nx, 33, dx, 5, rng)
+    bc = "impedance"
+    _, near = _sweep(g, bc, n, probes, mode, chunk, True)
+    _, full = _sweep(g, bc, n, probes, mode, chunk, False)
+    assert rel_l2(near.cpu().numpy(), full.cpu().numpy()) < 1e-6
+    if mode != "exact_adjoint":
+        ref = pwe.solve(g, bc, n, probes, mode)  # (P, nx, nz)
+        got = near.cpu().numpy().transpose(0, 2, 1)
+        if mode == "adjoint":
+            got = got[:, :, ::-1]
+        assert rel_l2(got, ref) < 1e-4
+
+
+def test_shared_status_covers_every_window(gpu, rng):
+    """Per-probe factor sets stepped in one sweep share one status: its reach is
+    the largest of the windows', so a strongly coupled window keeps the full
+    carries for all."""
+    k = 2 * np.pi
+    jobs, C_list = [], []
+    for dx in (2.0, 0.25):
+        g = pwe.Grid(dimension=2, nx=512, nz=9, dx=dx, dz=4.0, k=k)
+        A, B = operators.step_operators_1d(512, k, dx, 4.0, "impedance")
+        C = torch.from_numpy(pwe.object_contribution(g, random_object(512, 9, rng, amp=1e-3))).to("cuda")
+        jobs.append((C, A, B, "forward"))
+    own = [tri_engine.factor_many([j], 8)[0] for j in jobs]
+    shared = tri_engine.new_status("cuda")
+    facs = tri_engine.factor_many(jobs, 8, status=shared)
+    reach = [int(f.status[1].item()) for f in own]
+    assert reach[0] < 8 <= reach[1]
+    assert int(shared[1].item()) == max(reach)
+    assert all(f.status is shared for f in facs)

@@ -82,6 +82,7 @@ class TriScan(ctypes.Structure):
         ("grad_skip_row0", c_int),
         ("coef_pstride", c_int64),
         ("chunk", c_int),
+        ("reach", c_void_p),
     ]
 
 
@@ -251,7 +252,7 @@ class LibraryMissing(RuntimeError):
     pass
 
 
-ABI_VERSION = 2  # TP_ABI_VERSION of include/thick_ptycho_amd.h
+ABI_VERSION = 3  # TP_ABI_VERSION of include/thick_ptycho_amd.h
 
 
 def load() -> ctypes.CDLL:

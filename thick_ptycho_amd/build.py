@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtp_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["tp_status.cpp", "tp_tri.hip", "tp_triw.hip", "tp_ms.hip", "tp_fftany.hip", "tp_band.hip", "tp_learn.hip", "tp_krylov.hip", "tp_ms2d.hip",
+SOURCES = ["tp_status.cpp", "tp_tri.hip", "tp_tri_sweep_c16.hip", "tp_tri_sweep_c32.hip", "tp_tri_sweep_c8.hip",
+           "tp_tri_sweep_c4.hip", "tp_tri_sweep_c2.hip", "tp_tri_sweep_c1.hip", "tp_triw.hip", "tp_ms.hip", "tp_fftany.hip", "tp_band.hip", "tp_learn.hip", "tp_krylov.hip", "tp_ms2d.hip",
            "tp_pint.hip", "tp_pie.hip", "tp_window.hip", "tp_filter.hip", "tp_refine.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

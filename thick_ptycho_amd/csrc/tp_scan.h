@@ -8,6 +8,19 @@ namespace tp {
 // Per-row coefficient arrays of one step, in this order.
 enum { A_L = 0, A_GF = 1, A_PINV = 2, A_Q = 3, A_GB = 4, A_SG = 5, A_COUNT = 6 };
 
+// Coupling reach.  The carry into a chunk from the previous one enters row m
+// with the factor gf_m = prod_{k <= m} (-l_k), and the carry from the next one
+// with gb_m = prod_{k >= m} q_k.  The factor kernels record, per factor set,
+// the largest number of rows of a chunk (from its first row forward, or its
+// last row backward) whose factor exceeds kCarryTau in |re| + |im|.  When that
+// reach is below the chunk, every chunk's total product is below kCarryTau,
+// so the carry into chunk l is the neighbouring chunk's local end value alone
+// (the terms dropped pass through at least one whole chunk) and only the
+// first/last `reach` rows take it.  kCarryTau = 2^-32 is 256 times below the
+// FP32 unit roundoff: the dropped terms are smaller than the rounding of one
+// addition on those rows.
+constexpr double kCarryTau = 0x1p-32;
+
 // Arguments of one per-step factorisation job (see tp_tri_factor).
 struct FactorArgs {
   const double2* C;

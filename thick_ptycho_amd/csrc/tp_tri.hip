@@ -26,7 +26,7 @@
 // combined by wavefront scans of affine maps (DPP row shifts), and a register
 // pass applies them.  The persistent kernel walks every step in one launch
 // with the state in registers; the plain kernel advances one step per launch.
-#include "tp_scan.h"
+#include "tp_tri_step.h"
 
 namespace tp {
 
@@ -103,6 +103,7 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
   double2 q[CH];
   double2 gf = one;
   bool bad = false;
+  int reach = 0;  // coupling reach of this lane's chunk (see below)
 #pragma unroll
   for (int m = 0; m < CH; ++m) {
     const int i = row0 + m;
@@ -114,6 +115,7 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
       const double2 pinv = zinv(piv);
       q[m] = (i == n - 1) ? zero : zscale(zmul(lsup[i], pinv), -1.0);
       gf = zmul(gf, zscale(l, -1.0));
+      if (zabs1(gf) > kCarryTau) reach = m + 1;
       o[A_L * NP + pi] = to_f2(l);
       o[A_GF * NP + pi] = to_f2(gf);
       o[A_PINV * NP + pi] = to_f2(pinv);
@@ -137,9 +139,16 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
 #pragma unroll
   for (int m = CH - 1; m >= 0; --m) {
     gb = zmul(gb, q[m]);
+    if (zabs1(gb) > kCarryTau) reach = max(reach, CH - m);
     o[A_GB * NP + m * 64 + lane] = to_f2(gb);
   }
-  if (bad && f.status) f.status[0] = 1;
+  if (f.status) {
+    if (bad) f.status[0] = 1;
+    // one atomic per wavefront: the largest reach over its lanes
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) reach = max(reach, __shfl_xor(reach, off));
+    if (lane == 0) atomicMax(f.status + 1, reach);
+  }
 }
 
 template <int CH>
@@ -164,443 +173,6 @@ __global__ __launch_bounds__(64 * FACTOR_WAVES) void tri_factor_batch_kernel(con
   for (int i = 1; i < TP_MAX_SCANS; ++i)
     if (i < b.njobs && (int)blockIdx.x >= b.blk_start[i]) j = i;
   tri_factor_body<CH>(b.jobs[j], blockIdx.x - b.blk_start[j], sh);
-}
-
-// One CN step for the CH rows of this lane:  u <- A^{-1}(sigma u + b) - u.
-// `cf` points at the step's coefficient arrays (LDS or global, lane-interleaved).
-// Without SIGMA (sigma = 2):  r = u + b/2 and u <- 2 x - u.
-// With SIGMA:                 r = sigma u + b and u <- x - u.
-struct NoHook {
-  __device__ __forceinline__ void operator()(int) const {}
-};
-
-// Direct form: every coefficient is read where it is used (the register
-// budget of CH = 32 holds no second bank).
-template <int CH, bool HAS_RHS, bool SIGMA, typename Ptr, typename Hook>
-__device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
-                                                     const Hook& hook) {
-  constexpr int NP = 64 * CH;
-  float2 y[CH];
-#pragma unroll
-  for (int m = 0; m < CH; ++m) {
-    float2 r = u[m];
-    if (SIGMA) r = cmul(cf[A_SG * NP + m * 64 + lane], r);
-    if (HAS_RHS) r = SIGMA ? cadd(r, rhs[m]) : make_float2(fmaf(0.5f, rhs[m].x, r.x), fmaf(0.5f, rhs[m].y, r.y));
-    y[m] = r;
-  }
-  hook(0);
-#pragma unroll
-  for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
-  hook(1);
-  {
-    float2 A = cf[A_GF * NP + (CH - 1) * 64 + lane];
-    float2 B = y[CH - 1];
-    scan_affine_up(A, B);
-    const float2 yin = from_prev_lane(B);
-#pragma unroll
-    for (int m = 0; m < CH; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
-  }
-#pragma unroll
-  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
-#pragma unroll
-  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
-  hook(2);
-  {
-    float2 A = cf[A_GB * NP + lane];
-    float2 B = y[0];
-    scan_affine_down(A, B, lane);
-    const float2 xin = from_next_lane(B);
-#pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const float2 x = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
-      u[m] = SIGMA ? csub(x, u[m]) : make_float2(fmaf(2.f, x.x, -u[m].x), fmaf(2.f, x.y, -u[m].y));
-    }
-  }
-  hook(3);
-}
-
-// Staged form (CH <= 16): the coefficients of each phase are read into
-// registers one phase ahead of their use (two banks of CH values), and
-// scheduling barriers keep the compiler from sinking the reads next to their
-// uses, so a phase waits on reads issued a phase earlier instead of on one read
-// every few instructions.  This pays where a SIMD holds one wavefront and
-// nothing else covers the waits; with two wavefronts per SIMD the direct form's
-// freer schedule is faster.
-template <int CH, bool HAS_RHS, bool SIGMA, bool STAGED = true, typename Ptr, typename Hook = NoHook>
-__device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
-                                              const Hook& hook = Hook()) {
-  if constexpr (CH > 16 || !STAGED) {
-    tri_step_lane_direct<CH, HAS_RHS, SIGMA>(u, cf, rhs, lane, hook);
-    return;
-  }
-  constexpr int NP = 64 * CH;
-  float2 y[CH], c1[CH], c2[CH];
-#pragma unroll
-  for (int m = 0; m < CH; ++m) {
-    c1[m] = cf[A_L * NP + m * 64 + lane];
-    c2[m] = cf[A_GF * NP + m * 64 + lane];
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) {
-    float2 r = u[m];
-    if (SIGMA) r = cmul(cf[A_SG * NP + m * 64 + lane], r);
-    if (HAS_RHS) r = SIGMA ? cadd(r, rhs[m]) : make_float2(fmaf(0.5f, rhs[m].x, r.x), fmaf(0.5f, rhs[m].y, r.y));
-    y[m] = r;
-  }
-  hook(0);
-#pragma unroll
-  for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
-  hook(1);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
-  __builtin_amdgcn_sched_barrier(0);
-  {
-    float2 A = c2[CH - 1];
-    float2 B = y[CH - 1];
-    scan_affine_up(A, B);
-    const float2 yin = from_prev_lane(B);
-#pragma unroll
-    for (int m = 0; m < CH; ++m) y[m] = cfma(c2[m], yin, y[m]);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], c1[m]);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
-  hook(2);
-  {
-    float2 A = c1[0];
-    float2 B = y[0];
-    scan_affine_down(A, B, lane);
-    const float2 xin = from_next_lane(B);
-#pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const float2 x = cfma(c1[m], xin, y[m]);
-      u[m] = SIGMA ? csub(x, u[m]) : make_float2(fmaf(2.f, x.x, -u[m].x), fmaf(2.f, x.y, -u[m].y));
-    }
-  }
-  hook(3);
-}
-
-struct SweepArgs {
-  tp_tri_scan scans[TP_MAX_SCANS];
-  int blk_start[TP_MAX_SCANS + 1];
-  int nscans;
-  int ppb;
-  int nblk;
-};
-
-__device__ __forceinline__ int find_scan(const SweepArgs& a, int blk) {
-  int sc = 0;
-#pragma unroll
-  for (int i = 1; i < TP_MAX_SCANS; ++i)
-    if (i < a.nscans && blk >= a.blk_start[i]) sc = i;
-  return sc;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical blocks so that the probes of one scan (which share
-// every step's coefficients) meet in as few L2 caches as possible.
-__device__ __forceinline__ int xcd_block(int hw, int nblk) {
-  const int q = nblk >> 3, r = nblk & 7;
-  const int x = hw & 7, j = hw >> 3;
-  return x * q + min(x, r) + j;
-}
-
-template <int CH>
-__device__ __forceinline__ void store_rows(float2* o, const float2 (&u)[CH], int row0, int n) {
-  if (row0 + CH <= n) {
-#pragma unroll
-    for (int m = 0; m < CH; ++m) o[row0 + m] = u[m];
-  } else {
-#pragma unroll
-    for (int m = 0; m < CH; ++m)
-      if (row0 + m < n) o[row0 + m] = u[m];
-  }
-}
-
-template <int CH>
-__device__ __forceinline__ void store_plane(const tp_tri_scan& d, int p, int s, const float2 (&u)[CH], int row0) {
-  const int n = d.n;
-  if (d.out)
-    store_rows<CH>((float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride,
-                   u, row0, n);
-  if (d.ckpt && (s % d.ckpt_every) == 0)
-    store_rows<CH>((float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride, u,
-                   row0, n);
-  if (d.exitp && s == d.steps) store_rows<CH>((float2*)d.exitp + (size_t)p * d.exit_pstride, u, row0, n);
-}
-
-// Coalesced plane stores: the wavefront's plane goes through an LDS tile
-// (row i at i + 2*(i/16): two pad words per 16 rows against bank conflicts,
-// row pairs stay 16-byte aligned) and leaves as 16-byte stores, each
-// instruction writing 64 consecutive row pairs (1 KiB).
-template <int CH>
-__device__ __forceinline__ void tile_put(float2* tile, const float2 (&u)[CH], int row0) {
-#pragma unroll
-  for (int m = 0; m < CH; ++m) tile[tile_index(row0 + m)] = u[m];
-  __builtin_amdgcn_wave_barrier();
-}
-
-template <int CH>
-struct PlaneWriter {
-  float2* dst[3];
-  const float2* tile;
-  int n, lane;
-
-  __device__ __forceinline__ void init(const tp_tri_scan& d, int p, int s, bool act, const float2* t, int ln) {
-    tile = t;
-    n = d.n;
-    lane = ln;
-    dst[0] = (act && d.out)
-                 ? (float2*)d.out + (size_t)p * d.out_pstride + (int64_t)(d.out_z0 + d.out_zdir * s) * d.out_zstride
-                 : nullptr;
-    dst[1] = (act && d.ckpt && (s % d.ckpt_every) == 0)
-                 ? (float2*)d.ckpt + (size_t)p * d.ckpt_pstride + (int64_t)(s / d.ckpt_every) * d.ckpt_zstride
-                 : nullptr;
-    dst[2] = (act && d.exitp && s == d.steps) ? (float2*)d.exitp + (size_t)p * d.exit_pstride : nullptr;
-  }
-
-  // group g of 4: row pairs j = 2 (lane + 64 k) for k in [g*CH/8, (g+1)*CH/8),
-  // each read from the tile just before its store
-  __device__ __forceinline__ void operator()(int g) const {
-    constexpr int KP = CH / 2 > 0 ? CH / 2 : 1;  // pair-instructions per plane
-    const int k0 = (g * KP) / 4, k1 = ((g + 1) * KP) / 4;
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      float2* o = dst[t];
-      if (o == nullptr) continue;
-      const bool aligned = (reinterpret_cast<uintptr_t>(o) & 15) == 0;
-      for (int k = k0; k < k1; ++k) {
-        const int j = 2 * (lane + 64 * k);
-        if (aligned && j + 1 < n) {
-          typedef float f4 __attribute__((ext_vector_type(4)));
-          *reinterpret_cast<f4*>(o + j) = *reinterpret_cast<const f4*>(tile + tile_index(j));
-        } else {
-          if (j < n) o[j] = tile[tile_index(j)];
-          if (j + 1 < n) o[j + 1] = tile[tile_index(j + 1)];
-        }
-      }
-    }
-  }
-  __device__ __forceinline__ void all() const {
-    (*this)(0);
-    (*this)(1);
-    (*this)(2);
-    (*this)(3);
-  }
-
-  // Batched form: every LDS read of the plane is issued before the first store,
-  // so the stores wait on the LDS once rather than once each (faster where one
-  // wavefront per SIMD leaves the waits exposed, slower in full launches).
-  __device__ __forceinline__ void all_batched() const {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    constexpr int NP = 64 * CH;
-    constexpr int KP = CH / 2 > 0 ? CH / 2 : 1;  // pair-instructions per plane
-    if (dst[0] == nullptr && dst[1] == nullptr && dst[2] == nullptr) return;
-    f4 v[KP];
-#pragma unroll
-    for (int k = 0; k < KP; ++k) {
-      const int j = 2 * (lane + 64 * k);
-      v[k] = (CH >= 2 || j < NP) ? *reinterpret_cast<const f4*>(tile + tile_index(j)) : f4{0.f, 0.f, 0.f, 0.f};
-    }
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      float2* o = dst[t];
-      if (o == nullptr) continue;
-      const bool aligned = (reinterpret_cast<uintptr_t>(o) & 15) == 0;
-#pragma unroll
-      for (int k = 0; k < KP; ++k) {
-        const int j = 2 * (lane + 64 * k);
-        if (aligned && j + 1 < n) {
-          *reinterpret_cast<f4*>(o + j) = v[k];
-        } else {
-          if (j < n) o[j] = make_float2(v[k].x, v[k].y);
-          if (j + 1 < n) o[j + 1] = make_float2(v[k].z, v[k].w);
-        }
-      }
-    }
-  }
-};
-
-// ------------------------------------------------------------------ persistent sweep
-
-// One workgroup = PPB wavefronts = PPB probes of one scan walking all steps in
-// lockstep.  The step's coefficients are staged in LDS and shared by the PPB
-// wavefronts; the next step's block is fetched into registers while the
-// current step computes, and the plane computed in step s is written at the
-// start of step s+1 (before that fetch) so the stores drain behind the math.
-template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
-__device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk, const int sc) {
-  constexpr int NP = 64 * CH;
-  constexpr int NT = 64 * PPB;
-  constexpr int NA = SIGMA ? 6 : 5;
-  constexpr int NV = NA * NP / 2;             // 16-byte vectors staged per step
-  constexpr int STRIDE_V = A_COUNT * NP / 2;  // 16-byte vectors per step in memory
-  constexpr int PER = (NV + NT - 1) / NT;
-  extern __shared__ f4v smem_v[];
-  const float2* cf_l = reinterpret_cast<const float2*>(smem_v);
-  constexpr int TILE = NP + NP / 8;  // per-wavefront store tile (float2)
-
-  const tp_tri_scan& d = a.scans[sc];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int p = (blk - a.blk_start[sc]) * PPB + wave;
-  const bool act = p < d.probes;
-  const int n = d.n;
-  const int L = d.steps;
-  const int row0 = lane * CH;
-  // per-probe factors (one probe per workgroup) or the scan's shared factors
-  const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef) + (PPB == 1 ? (size_t)(act ? p : 0) * (d.coef_pstride / 2) : 0);
-  float2* tile = reinterpret_cast<float2*>(smem_v) + NA * NP + wave * TILE;
-  // gradient row accumulated over the workgroup's probes (tile_index layout)
-  float2* gbuf = reinterpret_cast<float2*>(smem_v) + NA * NP + PPB * TILE;
-
-  f4v st[PER];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int idx = threadIdx.x + k * NT;
-    if (NV % NT == 0 || idx < NV) st[k] = coef4[idx];
-  }
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int idx = threadIdx.x + k * NT;
-    if (NV % NT == 0 || idx < NV) smem_v[idx] = st[k];
-  }
-
-  // The initial plane arrives through the LDS tile: coalesced, and the state
-  // registers never hold a pending global load when the step loop starts
-  // (otherwise the first use of u in every iteration waits on vmcnt(0)).
-  float2 u[CH];
-  {
-    const float2* u0 = (const float2*)d.u0 + (size_t)(act ? p : 0) * d.u0_pstride;
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      const int j = lane + 64 * k;
-      tile[tile_index(j)] = (act && j < n) ? u0[j] : make_float2(0.f, 0.f);
-    }
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int m = 0; m < CH; ++m) u[m] = tile[tile_index(row0 + m)];
-    __builtin_amdgcn_wave_barrier();
-  }
-  __syncthreads();
-
-  float2 ga[CH], gb[CH];
-  if (GRAD) {
-    for (int idx = threadIdx.x; idx < TILE; idx += NT) gbuf[idx] = make_float2(0.f, 0.f);
-    const float2* g = (const float2*)d.gfwd + (size_t)(act ? p : 0) * d.gfwd_pstride + (int64_t)d.gfwd_z0 * d.gfwd_zstride;
-#pragma unroll
-    for (int m = 0; m < CH; ++m) ga[m] = (act && row0 + m < n) ? g[row0 + m] : make_float2(0.f, 0.f);
-    __syncthreads();
-  }
-
-  float2 rb[CH];
-  for (int s = 1; s <= L; ++s) {
-    // Plane s-1 is written first, then the next step's coefficients are
-    // fetched: vmcnt retires in order, so the wait for that fetch at the end
-    // of the step only covers stores that have had a whole step to drain.
-    {
-      tile_put<CH>(tile, u, row0);
-      PlaneWriter<CH> w;
-      w.init(d, p, s - 1, act, tile, lane);
-      if constexpr (STAGED) w.all_batched(); else w.all();
-    }
-    if (s < L) {
-      const f4v* src = coef4 + (size_t)s * STRIDE_V;
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int idx = threadIdx.x + k * NT;
-        if (NV % NT == 0 || idx < NV) st[k] = src[idx];
-      }
-    }
-    if (GRAD) {  // plane b = u_{s-1} of this step (prefetched before the math)
-      const float2* g =
-          (const float2*)d.gfwd + (size_t)(act ? p : 0) * d.gfwd_pstride + (int64_t)(d.gfwd_z0 + d.gfwd_zdir * s) * d.gfwd_zstride;
-#pragma unroll
-      for (int m = 0; m < CH; ++m) gb[m] = (act && row0 + m < n) ? g[row0 + m] : make_float2(0.f, 0.f);
-    }
-    if (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps)) {
-      const float2* r = (const float2*)d.rhs + (size_t)(act ? p : 0) * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
-#pragma unroll
-      for (int m = 0; m < CH; ++m) rb[m] = (act && row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-      tri_step_lane<CH, true, SIGMA, STAGED>(u, cf_l, rb, lane);
-    } else {
-      tri_step_lane<CH, false, SIGMA, STAGED>(u, cf_l, rb, lane);
-    }
-    int grow = 0;
-    if (GRAD) {
-      grow = d.grad_z0 + d.grad_zdir * (s - 1);
-      const bool skip_b = d.grad_skip_row0 && grow == 0;
-      if (act) {
-#pragma unroll
-        for (int m = 0; m < CH; ++m) {
-          const float2 w = skip_b ? ga[m] : cadd(ga[m], gb[m]);
-          const float2 c = cmul(cconj(u[m]), w);
-          float2* slot = gbuf + tile_index(row0 + m);
-          atomicAdd(&slot->x, c.x);
-          atomicAdd(&slot->y, c.y);
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < CH; ++m) ga[m] = gb[m];
-    }
-    __syncthreads();
-    if (GRAD) {  // one global atomic per row and component per workgroup
-      float* gr = (float*)((float2*)d.grad + (int64_t)grow * d.grad_zstride);
-      for (int i = threadIdx.x; i < n; i += NT) {
-        const float2 v = gbuf[tile_index(i)];
-        unsafeAtomicAdd(gr + 2 * i, v.x);
-        unsafeAtomicAdd(gr + 2 * i + 1, v.y);
-        gbuf[tile_index(i)] = make_float2(0.f, 0.f);
-      }
-    }
-    if (s < L) {
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int idx = threadIdx.x + k * NT;
-        if (NV % NT == 0 || idx < NV) smem_v[idx] = st[k];
-      }
-    }
-    __syncthreads();
-  }
-  tile_put<CH>(tile, u, row0);
-  {
-    PlaneWriter<CH> w;
-    w.init(d, p, L, act, tile, lane);
-    if constexpr (STAGED) w.all_batched(); else w.all();
-  }
-}
-
-// A workgroup's probes all belong to one scan, so the choice between the
-// launch's chunk and half of it (tp_tri_scan.chunk) is uniform per workgroup;
-// the LDS is sized for the launch's chunk and the narrow body uses a prefix.
-template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
-__global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) {
-  // Blocks in launch order: the dispatcher spreads consecutive workgroups over
-  // the XCDs and CUs, so the first workgroups of the launch (the first scans)
-  // get a CU each before any CU takes a second one.  Grouping each XCD's blocks
-  // into one contiguous range instead (coefficient reuse in one L2) measured
-  // 3-4 % slower on the config-1 launch (4.07-4.21 ms against 3.95-4.08 ms):
-  // it stacks two wide-scan workgroups on the same CUs.
-  const int blk = (int)blockIdx.x;
-  const int sc = find_scan(a, blk);
-  if constexpr (!GRAD && CH >= 16) {
-    if (a.scans[sc].chunk == CH / 2) {
-      tri_sweep_body<CH / 2, PPB, SIGMA, GRAD, STAGED>(a, blk, sc);
-      return;
-    }
-  }
-  tri_sweep_body<CH, PPB, SIGMA, GRAD, STAGED>(a, blk, sc);
 }
 
 // ------------------------------------------------------------------ plain one-step
@@ -682,7 +254,7 @@ static int check_chunk(int chunk) {
 }
 
 static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, SweepArgs& a, int& sigma,
-                      bool allow_half_chunk) {
+                      bool allow_half_chunk, bool widest_first = false) {
   if (nscans < 1 || nscans > TP_MAX_SCANS) {
     set_error("nscans must be in [1, %d] (got %d)", TP_MAX_SCANS, nscans);
     return TP_ERR_INVALID;
@@ -731,77 +303,37 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
     a.blk_start[i] = nblk;
     nblk += (d.probes + ppb - 1) / ppb;
   }
+  if (widest_first) {
+    // Workgroups in launch order take one compute unit each before any takes a
+    // second: the widest scans first, so that a unit that takes two pairs a
+    // wide workgroup with a narrower one (stable order among equal widths).
+    tp_tri_scan sorted[TP_MAX_SCANS];
+    int m = 0;
+    for (int w = 64 * 32; w >= 1 && m < nscans; --w)
+      for (int i = 0; i < nscans; ++i)
+        if (a.scans[i].n == w) sorted[m++] = a.scans[i];
+    nblk = 0;
+    for (int i = 0; i < nscans; ++i) {
+      a.scans[i] = sorted[i];
+      a.blk_start[i] = nblk;
+      nblk += (sorted[i].probes + ppb - 1) / ppb;
+    }
+  }
   a.blk_start[nscans] = nblk;
   a.nscans = nscans;
   a.ppb = ppb;
   a.nblk = nblk;
+  a.nblk_hw = nblk;
+  a.cus_per_xcd = 0;
+  const int cus = widest_first ? device_cus() : 0;
+  if (cus >= 8 && cus % 8 == 0 && nblk > 0) {  // XCD-aware order (sweep_block)
+    const int C = cus / 8;
+    const int rounds = (nblk + 8 * C - 1) / (8 * C);
+    const int last = nblk - (rounds - 1) * 8 * C;
+    a.cus_per_xcd = C;
+    a.nblk_hw = 8 * ((rounds - 1) * C + (last + 7) / 8);
+  }
   return TP_OK;
-}
-
-// Compute units of the current device (cached per device).
-static int device_cus() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cus[dev] = n;
-  }
-  return cus[dev];
-}
-
-template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED>
-static int launch_sweep_form(const SweepArgs& a, size_t lds, hipStream_t st) {
-  if (lds > 65536) {
-    TP_CHECK_HIP(hipFuncSetAttribute((const void*)tri_sweep_kernel<CH, PPB, SIGMA, GRAD, STAGED>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
-  hipLaunchKernelGGL((tri_sweep_kernel<CH, PPB, SIGMA, GRAD, STAGED>), dim3(a.nblk), dim3(64 * PPB), lds, st, a);
-  TP_CHECK_HIP(hipGetLastError());
-  return TP_OK;
-}
-
-// Launches of at most one workgroup per compute unit leave one wavefront per
-// SIMD and take the staged step; fuller launches take the direct one.
-template <int CH, int PPB, bool SIGMA, bool GRAD>
-static int launch_sweep(const SweepArgs& a, hipStream_t st) {
-  constexpr int NP = 64 * CH;
-  const size_t lds =
-      ((size_t)(SIGMA ? 6 : 5) * NP + (size_t)(PPB + (GRAD ? 1 : 0)) * (NP + NP / 8)) * sizeof(float2);
-  if (lds > 163840) {
-    set_error("chunk %d with %d probes per block needs %zu bytes of LDS (> 160 KiB); use fewer probes per block",
-              CH, PPB, lds);
-    return TP_ERR_INVALID;
-  }
-  if constexpr (CH <= 16) {
-    if (a.nblk <= device_cus()) return launch_sweep_form<CH, PPB, SIGMA, GRAD, true>(a, lds, st);
-  }
-  return launch_sweep_form<CH, PPB, SIGMA, GRAD, false>(a, lds, st);
-}
-
-template <int CH, bool SIGMA>
-static int sweep_ppb(const SweepArgs& a, int ppb, bool grad, hipStream_t st) {
-  if (grad) {  // gradient accumulation is built for 1 and 4 probes per workgroup
-    if (ppb == 1) return launch_sweep<CH, 1, SIGMA, true>(a, st);
-    if (ppb == 4) return launch_sweep<CH, 4, SIGMA, true>(a, st);
-    set_error("gradient sweeps need probes_per_block 1 or 4 (got %d)", ppb);
-    return TP_ERR_INVALID;
-  }
-  switch (ppb) {
-    case 1: return launch_sweep<CH, 1, SIGMA, false>(a, st);
-    case 2: return launch_sweep<CH, 2, SIGMA, false>(a, st);
-    case 4: return launch_sweep<CH, 4, SIGMA, false>(a, st);
-    case 8: return launch_sweep<CH, 8, SIGMA, false>(a, st);
-    default:
-      set_error("probes_per_block must be 1, 2, 4 or 8 (got %d)", ppb);
-      return TP_ERR_INVALID;
-  }
-}
-
-template <int CH>
-static int sweep_sigma(const SweepArgs& a, int ppb, int sigma, bool grad, hipStream_t st) {
-  return sigma ? sweep_ppb<CH, true>(a, ppb, grad, st) : sweep_ppb<CH, false>(a, ppb, grad, st);
 }
 
 }  // namespace tp
@@ -914,18 +446,20 @@ extern "C" int tp_tri_sweep(const tp_tri_scan* scans, int nscans, int chunk, int
   if (rc) return rc;
   SweepArgs a;
   int sigma = 0;
-  rc = build_args(scans, nscans, chunk, probes_per_block, a, sigma, true);
+  rc = build_args(scans, nscans, chunk, probes_per_block, a, sigma, true, true);
   if (rc) return rc;
   if (a.nblk == 0) return TP_OK;
   hipStream_t st = (hipStream_t)stream;
   const bool grad = scans[0].grad != nullptr;
+  bool rhs = false;
+  for (int i = 0; i < nscans; ++i) rhs = rhs || scans[i].rhs != nullptr;
   switch (chunk) {
-    case 1: return sweep_sigma<1>(a, probes_per_block, sigma, grad, st);
-    case 2: return sweep_sigma<2>(a, probes_per_block, sigma, grad, st);
-    case 4: return sweep_sigma<4>(a, probes_per_block, sigma, grad, st);
-    case 8: return sweep_sigma<8>(a, probes_per_block, sigma, grad, st);
-    case 16: return sweep_sigma<16>(a, probes_per_block, sigma, grad, st);
-    case 32: return sweep_sigma<32>(a, probes_per_block, sigma, grad, st);
+    case 1: return tri_sweep_launch_c1(a, probes_per_block, sigma, grad, rhs, st);
+    case 2: return tri_sweep_launch_c2(a, probes_per_block, sigma, grad, rhs, st);
+    case 4: return tri_sweep_launch_c4(a, probes_per_block, sigma, grad, rhs, st);
+    case 8: return tri_sweep_launch_c8(a, probes_per_block, sigma, grad, rhs, st);
+    case 16: return tri_sweep_launch_c16(a, probes_per_block, sigma, grad, rhs, st);
+    case 32: return tri_sweep_launch_c32(a, probes_per_block, sigma, grad, rhs, st);
   }
   return TP_ERR_INVALID;
 }

@@ -132,6 +132,8 @@ __global__ __launch_bounds__(1024) void triw_factor_kernel(const FactorArgs f) {
     o[A_GB * NPW + m * T + g] = to_f2(gb);
   }
   if (bad && f.status) f.status[0] = 1;
+  // the wide sweep always carries across every chunk: record a full reach
+  if (f.status && threadIdx.x == 0) atomicMax(f.status + 1, 64);
 }
 
 // ------------------------------------------------------------------ sweep

@@ -253,11 +253,12 @@ class PWEIterativeLUSolver(BasePWESolver):
                 def contribution(s):
                     return self.object_contribution_device(n_p, s)
             coef = torch.empty((S, per), dtype=torch.complex64, device=self.device)
+            status = tri_engine.new_status(self.device)  # shared: the sweep steps all windows' sets together
             fac = None
             for i in range(0, S, 8):
                 jobs = [(contribution(s), fm.A, fm.B, mode) for s in range(i, min(i + 8, S))]
                 facs = tri_engine.factor_many(jobs, self.chunk, out=[coef[s] for s in range(i, min(i + 8, S))],
-                                              waves=self.waves if wide else None)
+                                              waves=self.waves if wide else None, status=status)
                 fac = fac or facs[0]
                 self._last_factors += facs
             whole = tri_engine.TriFactors(coef.reshape(-1), fac.n, L, self.chunk, mode, fac.b_sign, fac.status,

@@ -32,7 +32,7 @@ class TriFactors:
     chunk: int
     mode: str
     b_sign: float
-    status: torch.Tensor
+    status: torch.Tensor  # int32 [2]: zero-pivot flag, coupling reach (tp_tri_factor's d_status)
     sigma: bool
     keep: list = field(default_factory=list)
     waves: int = 1  # wavefronts per probe (wide layout only)
@@ -42,7 +42,7 @@ class TriFactors:
 
     def check(self) -> None:
         """Raise if a pivot was exactly zero (synchronises)."""
-        if int(self.status.item()) != 0:
+        if int(self.status[0].item()) != 0:
             raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
 
 
@@ -50,7 +50,7 @@ def check_pivots(facs) -> None:
     """Raise RuntimeError if any of the factor sets hit an exactly zero pivot
     (one synchronisation for all of them); the reference's splu raises there."""
     st = [f.status for f in facs if isinstance(f, TriFactors)]
-    if st and int(torch.stack([x.reshape(()) for x in st]).max().item()) != 0:
+    if st and int(torch.stack([x[0] for x in st]).max().item()) != 0:
         raise RuntimeError("singular per-slice factor (zero pivot) in tp_tri_factor")
 
 
@@ -102,7 +102,7 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
         coef = out
     else:
         coef = torch.empty(elems, dtype=torch.complex64, device=dev)
-    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    status = new_status(dev)
     lib = _lib.load()
     sh = _lib.stream_handle(stream)
     if wide:
@@ -123,20 +123,37 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
                       plan=plan)
 
 
+def new_status(device) -> torch.Tensor:
+    """Zeroed factor status words: [0] zero-pivot flag, [1] coupling reach
+    (the rows a carry reaches in a chunk; below the chunk the sweep skips the
+    64-lane carry scans)."""
+    return torch.zeros(2, dtype=torch.int32, device=device)
+
+
 def coef_elems(L: int, chunk: int, waves: int = 1) -> int:
     """Complex elements of one factor set of L slices (tp_tri_coef_bytes[_wide] / 8)."""
     return L * 6 * 64 * waves * chunk
 
 
-def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves: Optional[int] = None) -> list:
+def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves: Optional[int] = None,
+                status: Optional[torch.Tensor] = None) -> list:
     """``factor`` for up to eight independent (C, A, B, mode) jobs in one launch
     (tp_tri_factor_batch); returns their TriFactors in order.  ``out`` optionally
     gives each job's coefficient buffer (complex64, coef_elems(L, chunk, waves)
-    values).  ``waves`` selects the wide layout, factored one job per launch."""
+    values).  ``waves`` selects the wide layout, factored one job per launch.
+    ``status`` (from new_status) is shared by every job when given: per-probe
+    factor sets stepped in one sweep need the zero-pivot flag and the coupling
+    reach of all of them."""
     assert 1 <= len(jobs) <= _lib.TP_MAX_SCANS
     if waves is not None:
-        return [factor(C, A, B, mode, chunk=chunk, waves=waves, stream=stream, out=None if out is None else out[k])
+        facs = [factor(C, A, B, mode, chunk=chunk, waves=waves, stream=stream, out=None if out is None else out[k])
                 for k, (C, A, B, mode) in enumerate(jobs)]
+        if status is not None:  # the wide sweep ignores the reach; keep the shared zero-pivot flag
+            for f in facs:
+                status[0:1] = torch.maximum(status[0:1], f.status[0:1])
+                status[1] = 64
+                f.status = status
+        return facs
     facs, arr = [], (_lib.TriFactorJob * len(jobs))()
     for k, (C, A, B, mode) in enumerate(jobs):
         assert C.is_cuda and C.dtype == torch.complex128 and C.dim() == 2
@@ -148,13 +165,13 @@ def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves
             assert coef.dtype == torch.complex64 and coef.is_contiguous() and coef.numel() == coef_elems(L, chunk)
         else:
             coef = torch.empty(coef_elems(L, chunk), dtype=torch.complex64, device=C.device)
-        status = torch.zeros(1, dtype=torch.int32, device=C.device)
+        st = status if status is not None else new_status(C.device)
         j = arr[k]
-        j.C, j.base, j.coef, j.status = _lib.ptr(C), _lib.ptr(base), _lib.ptr(coef), _lib.ptr(status)
+        j.C, j.base, j.coef, j.status = _lib.ptr(C), _lib.ptr(base), _lib.ptr(coef), _lib.ptr(st)
         j.lhs_sign, j.rhs_sign = plan.lhs_sign, plan.rhs_sign
         j.n, j.ncol, j.steps, j.conj_c = n, L, L, plan.conj_c
         j.lhs_col0, j.rhs_col0, j.col_step, j.with_sigma = plan.lhs_col0, plan.rhs_col0, plan.col_step, int(plan.sigma)
-        facs.append(TriFactors(coef, n, L, chunk, mode, plan.b_sign, status, plan.sigma, keep=[base, C]))
+        facs.append(TriFactors(coef, n, L, chunk, mode, plan.b_sign, st, plan.sigma, keep=[base, C]))
     _lib.check(_lib.load().tp_tri_factor_batch(arr, len(jobs), chunk, _lib.stream_handle(stream)),
                "tp_tri_factor_batch")
     return facs
@@ -189,6 +206,7 @@ class TriScanSpec:
     grad_z0: int = 0
     grad_zdir: int = 1
     grad_skip_row0: bool = False
+    near: bool = True  # False: full 64-lane carries whatever the factors' coupling reach
 
     def struct(self) -> _lib.TriScan:
         f = self.fac
@@ -197,6 +215,8 @@ class TriScanSpec:
         s = _lib.TriScan()
         s.coef = _lib.ptr(f.coef)
         s.coef_pstride = int(self.coef_pstride)
+        if self.near and not f.wide and f.status.numel() >= 2:
+            s.reach = _lib.ptr(f.status) + f.status.element_size()
         s.sigma = int(f.sigma)
         assert self.u0.dtype == torch.complex64 and self.u0.shape[-1] == n and self.u0.stride(-1) == 1
         s.u0 = _lib.ptr(self.u0)

@@ -255,11 +255,12 @@ def amplitude_gradient_reduced(n, probes, sqrt_i, windows, k: float, dx: float, 
     whole = {}
     for mode in ("forward", "exact_adjoint"):
         coef = torch.empty((S, per), dtype=torch.complex64, device=device)
+        status = tri_engine.new_status(device)  # shared: one sweep steps every window's set
         first = None
         for i in range(0, S, _lib.TP_MAX_SCANS):
             jobs = [(C[a:b], A, B, mode) for a, b in win[i:i + _lib.TP_MAX_SCANS]]
             facs = tri_engine.factor_many(jobs, chunk, out=[coef[s] for s in range(i, min(i + _lib.TP_MAX_SCANS, S))],
-                                          waves=waves if wide else None)
+                                          waves=waves if wide else None, status=status)
             first = first or facs[0]
         whole[mode] = tri_engine.TriFactors(coef.reshape(-1), W, L, chunk, mode, first.b_sign, first.status,
                                             first.sigma, keep=[coef, facs], waves=waves, wide=wide)
