@@ -99,47 +99,22 @@ __global__ __launch_bounds__(64 * PPB) void ms_sweep_kernel(const MsArgs a) {
 // leaves SIMDs idle and is bound by the latency of its LDS stages).  With
 // M = n / S, n = n2 + M n1 and k = k1 + S k2,
 //   X[k1 + S k2] = sum_n2 W_M^{n2 k2} [ W_n^{n2 k1} sum_n1 W_S^{n1 k1} x[n2 + M n1] ],
-// so wavefront w keeps x[w M + n2] (contiguous loads and stores), an S-point
-// DFT across the wavefronts (through LDS) turns it into the k1 = w
-// sub-sequence, and a wave-level M-point FFT finishes the transform, leaving
-// X[w + S k2] at lane/register k2 = l + 64 m; the inverse runs the steps
+// so an S-point DFT over x[n2 + M n1] (n1 = 0..S-1), the twiddle W_n^{n2 k1}
+// and a wave-level M-point FFT of the k1 = w sub-sequence give X[w + S k2] at
+// lane/register k2 = l + 64 m of wavefront w; the inverse runs the steps
 // backwards with conjugate twiddles.
-template <int S, int RM, int SIGN>
-__device__ __forceinline__ void ms_cross(float2* x, float2* ta, float2* tb, const float2* twN, int w, int lane) {
-  constexpr int M = 64 * RM, N = S * M, QL = M / S / 64;
-  if (SIGN > 0) {  // inverse: twiddle W_n^{+n2 k1} (k1 = w) before the S-point DFT
-#pragma unroll
-    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], twid<+1>(twN, ((lane + 64 * m) * w) & (N - 1)));
-  }
-#pragma unroll
-  for (int m = 0; m < RM; ++m) ta[lp(w * M + lane + 64 * m)] = x[m];
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < QL; ++t) {
-    const int n2 = w * (M / S) + lane + 64 * t;
-    float2 v[S];
-#pragma unroll
-    for (int r = 0; r < S; ++r) v[r] = ta[lp(r * M + n2)];
-    dft_reg<S, SIGN>(v);
-    if (SIGN < 0) {  // forward: twiddle W_n^{n2 k1} after the S-point DFT
-#pragma unroll
-      for (int k = 1; k < S; ++k) v[k] = cmul(v[k], twid<-1>(twN, (n2 * k) & (N - 1)));
-    }
-#pragma unroll
-    for (int k = 0; k < S; ++k) tb[lp(k * M + n2)] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < RM; ++m) x[m] = tb[lp(w * M + lane + 64 * m)];
-}
-
-// ms_cross's twiddles, per lane and fixed for a sweep, held in registers:
-// fwd[t][k] = W_n^{n2 k} for the forward exchange (n2 = w M/S + lane + 64 t),
-// inv[m] = W_n^{(lane + 64 m) w} (conjugated when used) for the inverse.
+//
+// Between slices the field stays in the layout of the S-point DFTs: wavefront w
+// keeps x[n2 + M n1] for its own n2 (n2 = w M/S + l + 64 t), which is exactly
+// what its inverse S-point DFT produces and what the next slice's forward one
+// takes.  So the transmission multiply, the plane stores (S runs of 64
+// consecutive points per t, coalesced) and the next forward DFT need no
+// exchange, and a slice crosses the wavefronts twice through LDS (two
+// workgroup barriers) instead of four.
 template <int S, int RM>
 struct CrossTw {
-  float2 fwd[RM / S][S];
-  float2 inv[RM];
+  float2 fwd[RM / S][S];  // W_n^{n2 k}, n2 = w M/S + lane + 64 t
+  float2 inv[RM];         // W_n^{(lane + 64 m) w}, conjugated when used
 };
 
 template <int S, int RM>
@@ -155,45 +130,13 @@ __device__ __forceinline__ void cross_tw_load(CrossTw<S, RM>& c, const float2* t
   for (int m = 0; m < RM; ++m) c.inv[m] = twN[((lane + 64 * m) * w) & (N - 1)];
 }
 
-// ms_cross with the twiddles of `c` (same exchange and order).  Every access of
-// the exchange is to consecutive values, so the buffers take no padding.
-template <int S, int RM, int SIGN>
-__device__ __forceinline__ void ms_cross_rt(float2* x, float2* ta, float2* tb, const CrossTw<S, RM>& c, int w,
-                                            int lane) {
-  constexpr int M = 64 * RM, QL = M / S / 64;
-  if (SIGN > 0) {
-#pragma unroll
-    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], cconj(c.inv[m]));
-  }
-#pragma unroll
-  for (int m = 0; m < RM; ++m) ta[w * M + lane + 64 * m] = x[m];
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < QL; ++t) {
-    const int n2 = w * (M / S) + lane + 64 * t;
-    float2 v[S];
-#pragma unroll
-    for (int r = 0; r < S; ++r) v[r] = ta[r * M + n2];
-    dft_reg<S, SIGN>(v);
-    if (SIGN < 0) {
-#pragma unroll
-      for (int k = 1; k < S; ++k) v[k] = cmul(v[k], c.fwd[t][k]);
-    }
-#pragma unroll
-    for (int k = 0; k < S; ++k) tb[k * M + n2] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < RM; ++m) x[m] = tb[w * M + lane + 64 * m];
-}
-
 template <int R, int S>
 __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
-  constexpr int N = 64 * R, M = N / S, RM = R / S;
+  constexpr int N = 64 * R, M = N / S, RM = R / S, QL = M / S / 64;
   static_assert(RM == 4 && M % (64 * S) == 0, "split form: 256-point wave transforms");
   __shared__ float2 twN[N];
   __shared__ float2 twM[M];
-  __shared__ float2 ta[N + N / 16], tb[N + N / 16];
+  __shared__ float2 ta[N], tb[N];
   __shared__ float2 buf[S][2][ms_bufn<RM>()];
   const int blk = blockIdx.x;  // one probe per workgroup
   const int sc = ms_find_scan(a, blk);
@@ -208,47 +151,88 @@ __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
   cross_tw_load<S, RM>(ctw, twN, w, lane);
   Tw256 wtw;
   tw256_load(wtw, twM, lane);
-  float2 H[RM], T[RM], Tn[RM], x[RM];
+  // xs[t][r] = x[r M + n2(t)], n2(t) = w M/S + lane + 64 t
+  auto pos = [&](int t, int r) { return r * M + w * (M / S) + lane + 64 * t; };
+  float2 H[RM], x[RM], xs[QL][S], T[QL][S], Tn[QL][S];
   const float2* Hg = (const float2*)d.H;
-  const float2* Tg = (const float2*)d.T + w * M;
-  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride + w * M;
+  const float2* Tg = (const float2*)d.T;
+  const float2* u0 = (const float2*)d.u0 + (size_t)p * d.u0_pstride;
 #pragma unroll
   for (int m = 0; m < RM; ++m) {
     H[m] = Hg[w + S * (lane + 64 * m)];
     if (d.conj_h) H[m] = cconj(H[m]);
-    x[m] = u0[lane + 64 * m];
-    T[m] = Tg[lane + 64 * m];
   }
-  float2* outp = d.out ? (float2*)d.out + (size_t)p * d.out_pstride + w * M : nullptr;
+#pragma unroll
+  for (int t = 0; t < QL; ++t)
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+      xs[t][r] = u0[pos(t, r)];
+      T[t][r] = Tg[pos(t, r)];
+    }
+  float2* outp = d.out ? (float2*)d.out + (size_t)p * d.out_pstride : nullptr;
   if (outp) {
 #pragma unroll
-    for (int m = 0; m < RM; ++m) outp[lane + 64 * m] = x[m];
+    for (int t = 0; t < QL; ++t)
+#pragma unroll
+      for (int r = 0; r < S; ++r) outp[pos(t, r)] = xs[t][r];
   }
   for (int z = 0; z < d.steps; ++z) {
     if (z + 1 < d.steps) {
 #pragma unroll
-      for (int m = 0; m < RM; ++m) Tn[m] = Tg[(size_t)(z + 1) * N + lane + 64 * m];
-    }
+      for (int t = 0; t < QL; ++t)
 #pragma unroll
-    for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], T[m]);
-    ms_cross_rt<S, RM, -1>(x, ta, tb, ctw, w, lane);
+        for (int r = 0; r < S; ++r) Tn[t][r] = Tg[(size_t)(z + 1) * N + pos(t, r)];
+    }
+    // transmission, forward S-point DFTs and twiddles, into the k1 = w sub-sequences
+#pragma unroll
+    for (int t = 0; t < QL; ++t) {
+      float2 v[S];
+#pragma unroll
+      for (int r = 0; r < S; ++r) v[r] = cmul(xs[t][r], T[t][r]);
+      dft_reg<S, -1>(v);
+#pragma unroll
+      for (int k = 1; k < S; ++k) v[k] = cmul(v[k], ctw.fwd[t][k]);
+#pragma unroll
+      for (int k = 0; k < S; ++k) tb[pos(t, k)] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < RM; ++m) x[m] = tb[w * M + lane + 64 * m];
     fft_wave4_rt<-1>(x, buf[w][0], buf[w][1], wtw, lane);
 #pragma unroll
     for (int m = 0; m < RM; ++m) x[m] = cmul(x[m], H[m]);
     fft_wave4_rt<+1>(x, buf[w][0], buf[w][1], wtw, lane);
-    ms_cross_rt<S, RM, +1>(x, ta, tb, ctw, w, lane);
+#pragma unroll
+    for (int m = 0; m < RM; ++m) ta[w * M + lane + 64 * m] = cmul(x[m], cconj(ctw.inv[m]));
+    __syncthreads();
+    // inverse S-point DFTs back to x[r M + n2]
+#pragma unroll
+    for (int t = 0; t < QL; ++t) {
+      float2 v[S];
+#pragma unroll
+      for (int r = 0; r < S; ++r) v[r] = ta[pos(t, r)];
+      dft_reg<S, +1>(v);
+#pragma unroll
+      for (int k = 0; k < S; ++k) xs[t][k] = v[k];
+    }
     if (outp) {
       float2* o = outp + (int64_t)(z + 1) * d.out_zstride;
 #pragma unroll
-      for (int m = 0; m < RM; ++m) o[lane + 64 * m] = x[m];
+      for (int t = 0; t < QL; ++t)
+#pragma unroll
+        for (int r = 0; r < S; ++r) o[pos(t, r)] = xs[t][r];
     }
 #pragma unroll
-    for (int m = 0; m < RM; ++m) T[m] = Tn[m];
+    for (int t = 0; t < QL; ++t)
+#pragma unroll
+      for (int r = 0; r < S; ++r) T[t][r] = Tn[t][r];
   }
   if (d.exitp) {
-    float2* o = (float2*)d.exitp + (size_t)p * d.exit_pstride + w * M;
+    float2* o = (float2*)d.exitp + (size_t)p * d.exit_pstride;
 #pragma unroll
-    for (int m = 0; m < RM; ++m) o[lane + 64 * m] = x[m];
+    for (int t = 0; t < QL; ++t)
+#pragma unroll
+      for (int r = 0; r < S; ++r) o[pos(t, r)] = xs[t][r];
   }
 }
 
