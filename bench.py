@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # 4 probes per workgroup, the 384-wide scan at 8 rows per lane), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy in the same process
 # (profiles/tri_sweep_batched_cfg1_traffic.md).
-MEASURED_SWEEP_TRAFFIC = {(1024, 4, True): 16.40e9}
+MEASURED_SWEEP_TRAFFIC = {(1024, 4, True): 13.32e9}
 
 
 def log(msg: str) -> None:
