@@ -230,8 +230,9 @@ int tp_tri_step(const tp_tri_scan* scans, int nscans, int chunk, int s, void* st
  * (n <= 2048) do not hold. */
 size_t tp_tri_coef_bytes_wide(int n, int steps, int chunk, int waves);
 
-/* tp_tri_factor for the wide layout: one workgroup of 64 * waves lanes per step
- * (d_status[1] is raised to 64: the wide sweep carries across every chunk). */
+/* tp_tri_factor for the wide layout: one workgroup of 64 * waves lanes per step.
+ * d_status[1] takes the coupling reach as in tp_tri_factor (tp_tri_sweep_wide
+ * carries across every chunk whatever it is). */
 int tp_tri_factor_wide(const void* d_C, int n, int ncol, const void* d_base,
                        double lhs_sign, double rhs_sign, int conj_c,
                        int lhs_col0, int rhs_col0, int col_step, int steps,

@@ -325,14 +325,7 @@ static int build_args(const tp_tri_scan* scans, int nscans, int chunk, int ppb, 
   a.nblk = nblk;
   a.nblk_hw = nblk;
   a.cus_per_xcd = 0;
-  const int cus = widest_first ? device_cus() : 0;
-  if (cus >= 8 && cus % 8 == 0 && nblk > 0) {  // XCD-aware order (sweep_block)
-    const int C = cus / 8;
-    const int rounds = (nblk + 8 * C - 1) / (8 * C);
-    const int last = nblk - (rounds - 1) * 8 * C;
-    a.cus_per_xcd = C;
-    a.nblk_hw = 8 * ((rounds - 1) * C + (last + 7) / 8);
-  }
+  if (widest_first) sweep_xcd_order(a);
   return TP_OK;
 }
 

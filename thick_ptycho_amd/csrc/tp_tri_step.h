@@ -352,6 +352,42 @@ inline int device_cus() {
   return cus[dev];
 }
 
+// Logical workgroup of hardware workgroup h.  Workgroups are dispatched to the
+// 8 XCDs round-robin (h % 8), and within an XCD the j-th (j = h / 8) takes
+// compute unit j % C (C units per XCD) once every unit has one.  The logical
+// workgroups (scans in launch order, widest first) are dealt out in rounds of
+// 8 C: in each round every XCD takes a contiguous run of them, so that a
+// scan's workgroups, which read the same per-step coefficient blocks, share
+// few L2 caches, and each unit takes at most one workgroup per round.  The
+// last round may leave slots empty (-1).  C = 0: launch order.
+__device__ __forceinline__ int sweep_block(const SweepArgs& a, int h) {
+  const int C = a.cus_per_xcd;
+  if (C == 0) return h;
+  const int x = h & 7, j = h >> 3;
+  const int r = j / C, i = j - r * C;
+  const int first = r * 8 * C;
+  const int nr = min(a.nblk - first, 8 * C);
+  const int q = nr >> 3, rem = nr & 7;
+  const int cnt = q + (x < rem ? 1 : 0);
+  if (i >= cnt) return -1;
+  return first + x * q + min(x, rem) + i;
+}
+
+// Host side of sweep_block: with C compute units per XCD (device_cus() / 8),
+// the launch takes 8 * (full rounds * C + the last round's largest run) slots.
+inline void sweep_xcd_order(SweepArgs& a) {
+  a.nblk_hw = a.nblk;
+  a.cus_per_xcd = 0;
+  const int cus = device_cus();
+  if (cus >= 8 && cus % 8 == 0 && a.nblk > 0) {
+    const int C = cus / 8;
+    const int rounds = (a.nblk + 8 * C - 1) / (8 * C);
+    const int last = a.nblk - (rounds - 1) * 8 * C;
+    a.cus_per_xcd = C;
+    a.nblk_hw = 8 * ((rounds - 1) * C + (last + 7) / 8);
+  }
+}
+
 // The persistent sweep of one chunk (tp_tri_sweep_c<chunk>.hip).
 int tri_sweep_launch_c1(const SweepArgs& a, int ppb, int sigma, bool grad, bool rhs, hipStream_t st);
 int tri_sweep_launch_c2(const SweepArgs& a, int ppb, int sigma, bool grad, bool rhs, hipStream_t st);

@@ -212,27 +212,6 @@ __device__ __forceinline__ void tri_sweep_dispatch(const SweepArgs& a, const int
   tri_sweep_body<CH, PPB, SIGMA, GRAD, STAGED, false, RHS>(a, blk, sc, CH);
 }
 
-// Logical workgroup of hardware workgroup h.  Workgroups are dispatched to the
-// 8 XCDs round-robin (h % 8), and within an XCD the j-th (j = h / 8) takes
-// compute unit j % C (C units per XCD) once every unit has one.  The logical
-// workgroups (scans in launch order, widest first) are dealt out in rounds of
-// 8 C: in each round every XCD takes a contiguous run of them, so that a
-// scan's workgroups, which read the same per-step coefficient blocks, share
-// few L2 caches, and each unit takes at most one workgroup per round.  The
-// last round may leave slots empty (-1).  C = 0: launch order.
-__device__ __forceinline__ int sweep_block(const SweepArgs& a, int h) {
-  const int C = a.cus_per_xcd;
-  if (C == 0) return h;
-  const int x = h & 7, j = h >> 3;
-  const int r = j / C, i = j - r * C;
-  const int first = r * 8 * C;
-  const int nr = min(a.nblk - first, 8 * C);
-  const int q = nr >> 3, rem = nr & 7;
-  const int cnt = q + (x < rem ? 1 : 0);
-  if (i >= cnt) return -1;
-  return first + x * q + min(x, rem) + i;
-}
-
 // A workgroup's probes all belong to one scan, so the choice between the
 // launch's chunk and half of it (tp_tri_scan.chunk) is uniform per workgroup;
 // the LDS is sized for the launch's chunk and the narrow body uses a prefix.

@@ -94,6 +94,7 @@ __global__ __launch_bounds__(1024) void triw_factor_kernel(const FactorArgs f) {
   double2 q[CH];
   double2 gf = one;
   bool bad = false;
+  int reach = 0;  // coupling reach of this lane's chunk (tp_scan.h)
 #pragma unroll
   for (int m = 0; m < CH; ++m) {
     const int i = row0 + m;
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(1024) void triw_factor_kernel(const FactorArgs f) {
       const double2 pinv = zinv(piv);
       q[m] = (i == n - 1) ? zero : zscale(zmul(lsup[i], pinv), -1.0);
       gf = zmul(gf, zscale(l, -1.0));
+      if (zabs1(gf) > kCarryTau) reach = m + 1;
       o[A_L * NPW + pi] = to_f2(l);
       o[A_GF * NPW + pi] = to_f2(gf);
       o[A_PINV * NPW + pi] = to_f2(pinv);
@@ -129,11 +131,15 @@ __global__ __launch_bounds__(1024) void triw_factor_kernel(const FactorArgs f) {
 #pragma unroll
   for (int m = CH - 1; m >= 0; --m) {
     gb = zmul(gb, q[m]);
+    if (zabs1(gb) > kCarryTau) reach = max(reach, CH - m);
     o[A_GB * NPW + m * T + g] = to_f2(gb);
   }
-  if (bad && f.status) f.status[0] = 1;
-  // the wide sweep always carries across every chunk: record a full reach
-  if (f.status && threadIdx.x == 0) atomicMax(f.status + 1, 64);
+  if (f.status) {
+    if (bad) f.status[0] = 1;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) reach = max(reach, __shfl_xor(reach, off));
+    if (lane == 0) atomicMax(f.status + 1, reach);
+  }
 }
 
 // ------------------------------------------------------------------ sweep

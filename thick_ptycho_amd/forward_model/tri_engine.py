@@ -148,10 +148,9 @@ def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves
     if waves is not None:
         facs = [factor(C, A, B, mode, chunk=chunk, waves=waves, stream=stream, out=None if out is None else out[k])
                 for k, (C, A, B, mode) in enumerate(jobs)]
-        if status is not None:  # the wide sweep ignores the reach; keep the shared zero-pivot flag
+        if status is not None:  # one status word pair for all of them
             for f in facs:
-                status[0:1] = torch.maximum(status[0:1], f.status[0:1])
-                status[1] = 64
+                status.copy_(torch.maximum(status, f.status))
                 f.status = status
         return facs
     facs, arr = [], (_lib.TriFactorJob * len(jobs))()
