@@ -81,7 +81,7 @@
 extern "C" {
 #endif
 
-#define TP_ABI_VERSION 3
+#define TP_ABI_VERSION 4
 
 #define TP_OK 0
 #define TP_ERR_INVALID (-1)     /* bad argument or unsupported size      */
@@ -196,13 +196,23 @@ typedef struct tp_tri_scan {
   int chunk;
   /* Optional (NULL = full carries): d_status + 1 of the factorisation(s) that
    * wrote `coef` (all of them, for per-probe factors).  When *reach is below
-   * the scan's chunk (8 rows or more), tp_tri_sweep takes the near-carry step:
+   * the scan's chunk (8 rows or more) and has_offd is set, tp_tri_sweep takes
+   * the near-carry step:
    * the carry into a chunk is the neighbouring chunk's local end value, applied
    * to the first / last *reach rows only, with no 64-lane scan; the terms it
    * leaves out are below 2^-32 of the carried values.  Read on the device when
    * the sweep starts, so a sweep queued behind its factorisation needs no host
    * synchronisation.  tp_tri_step and tp_tri_sweep_wide ignore it. */
   const int* reach;
+  /* The LHS off-diagonals of the factored operator, which the near-carry step
+   * needs (it derives the multipliers l_i = sub_i / piv_{i-1} and
+   * q_i = -sup_i / piv_i from the stored 1/piv instead of staging them): with
+   * has_offd != 0, sub is uniform on rows 2 .. n-2 and sup on rows 1 .. n-3,
+   * and offd holds complex64 {sub interior, sup interior, sub[1], sub[n-1],
+   * sup[0], sup[n-2]} as 12 floats (every mode's operator has this form, n >= 4).
+   * has_offd = 0: no near-carry step. */
+  float offd[12];
+  int has_offd;
 } tp_tri_scan;
 
 /* Persistent batched sweep: one launch walks all `steps` slices of every probe

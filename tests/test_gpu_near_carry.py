@@ -3,7 +3,9 @@
 The factor kernel records the coupling reach of its factors (the rows over
 which a carry from a neighbouring 64-lane chunk keeps a factor above 2^-32);
 below the chunk, the sweep replaces the two 64-lane affine scans by the
-neighbouring chunk's local end value and applies it to `reach` rows only.
+neighbouring chunk's local end value and applies it to `reach` rows only, and
+derives the multipliers l and q from the stored 1/piv and the operator's
+off-diagonals instead of staging them.
 
 Checks: the reach equals the one of a sequential FP64 Thomas factorisation of
 the same per-slice operators; near and full carries give the same planes to
@@ -125,3 +127,29 @@ def test_shared_status_covers_every_window(gpu, rng):
     assert reach[0] < 8 <= reach[1]
     assert int(shared[1].item()) == max(reach)
     assert all(f.status is shared for f in facs)
+
+
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann", "impedance"])
+@pytest.mark.parametrize("mode", ["forward", "adjoint", "reverse", "exact_adjoint"])
+@pytest.mark.parametrize("nx", [1009, 1010, 1024, 641, 20])
+def test_boundary_rows_of_the_derived_multipliers(gpu, rng, bc, mode, nx):
+    """The near-carry step derives l and q from 1/piv with the operator's boundary
+    off-diagonals (sub at rows 1 and n-1, sup at rows 0 and n-2) placed at every
+    position in a lane: n - 1 on a chunk's first row (1009, 641), second row
+    (1010), last row (1024), and both ends in the first two lanes (20)."""
+    print(f"case boundary rows bc={bc} mode={mode} nx={nx}", flush=True)
+    g, n, probes = _case(nx, 9, 2.0, 3, rng)
+    if mode == "reverse" and bc == "impedance":
+        g = pwe.Grid(dimension=2, nx=nx, nz=5, dx=2.0, dz=4.0, k=g.k)
+        n = n[:, :5]
+    fac_n, near = _sweep(g, bc, n, probes, mode, 16, True)
+    _, full = _sweep(g, bc, n, probes, mode, 16, False)
+    assert int(fac_n.status[1].item()) < 16
+    assert tri_engine.near_offdiagonals(fac_n.plan, nx) is not None
+    assert rel_l2(near.cpu().numpy(), full.cpu().numpy()) < 1e-6
+    if mode != "exact_adjoint":
+        ref = pwe.solve(g, bc, n, probes, mode)
+        got = near.cpu().numpy().transpose(0, 2, 1)
+        if mode == "adjoint":
+            got = got[:, :, ::-1]
+        assert rel_l2(got, ref) < 1e-4

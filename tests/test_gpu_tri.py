@@ -120,7 +120,9 @@ def test_plain_equals_persistent_and_outputs(gpu, rng):
     s = _solver(nx, nz, 2.0, 4.0, 2 * np.pi, probes, "impedance")
     a = s.solve_device(n=n)
     b = s.solve_device(n=n, plain=True)
-    torch.testing.assert_close(a, b, rtol=0, atol=0)
+    # the persistent sweep's near-carry step forms the multipliers from 1/piv in
+    # FP32 where the plain form reads the stored ones: equal to rounding
+    assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
     e = s.solve_device(n=n, keep="exit")
     torch.testing.assert_close(e, a[:, :, :, -1], rtol=0, atol=0)
     # checkpoints every 8 planes equal the full field planes

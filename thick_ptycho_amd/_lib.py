@@ -83,6 +83,8 @@ class TriScan(ctypes.Structure):
         ("coef_pstride", c_int64),
         ("chunk", c_int),
         ("reach", c_void_p),
+        ("offd", ctypes.c_float * 12),
+        ("has_offd", c_int),
     ]
 
 
@@ -252,7 +254,7 @@ class LibraryMissing(RuntimeError):
     pass
 
 
-ABI_VERSION = 3  # TP_ABI_VERSION of include/thick_ptycho_amd.h
+ABI_VERSION = 4  # TP_ABI_VERSION of include/thick_ptycho_amd.h
 
 
 def load() -> ctypes.CDLL:

@@ -19,12 +19,56 @@ template <int CH>
 constexpr int kNearGroup = CH < 4 ? CH : 4;
 constexpr int kNearMinChunk = 8;
 
+// The near-carry step derives the multipliers from the pivots instead of
+// staging them: l_i = sub_i pinv_{i-1} and q_i = -sup_i pinv_i, with the LHS
+// off-diagonals uniform on interior rows (tp_tri_scan.offd).  Each array has one
+// boundary value at either end (rows 1 and n-1 of sub, 0 and n-2 of sup); a
+// lane keeps the values of its own rows: the first lane's, and at the uniform
+// rows mn = (n-1) % CH, mq = (n-2) % CH those of the lanes holding rows n-1 and
+// n-2.  A boundary row that starts or ends a chunk takes its multiplier through
+// the stored carry factors (gf, gb) instead.
+struct Offd {
+  float2 sub, sup;        // interior rows
+  float2 sub_x1, sup_x0;  // this lane at m = 1 (sub) / m = 0 (sup)
+  float2 sub_xn, sup_xn;  // this lane at m = mn (sub) / m = mq (sup)
+  int mn, mq;
+};
+
+__device__ __forceinline__ Offd make_offd(const tp_tri_scan& d, int chunk, int lane) {
+  const float* v = d.offd;
+  Offd o;
+  o.sub = make_float2(v[0], v[1]);
+  o.sup = make_float2(v[2], v[3]);
+  const float2 sub1 = make_float2(v[4], v[5]), subn = make_float2(v[6], v[7]);
+  const float2 sup0 = make_float2(v[8], v[9]), supn = make_float2(v[10], v[11]);
+  const int n = d.n;
+  o.mn = (n - 1) % chunk;
+  o.mq = n >= 2 ? (n - 2) % chunk : -1;
+  o.sub_x1 = lane == 0 ? sub1 : o.sub;
+  o.sup_x0 = lane == 0 ? sup0 : o.sup;
+  o.sub_xn = lane == (n - 1) / chunk ? subn : (o.mn == 1 ? o.sub_x1 : o.sub);
+  o.sup_xn = (n >= 2 && lane == (n - 2) / chunk) ? supn : (o.mq == 0 ? o.sup_x0 : o.sup);
+  return o;
+}
+
+// l_m and q_m of this lane's row m (m a compile-time index)
+__device__ __forceinline__ float2 offd_l(const Offd& o, int m, float2 pinv_prev) {
+  float2 a = (m == 1) ? o.sub_x1 : o.sub;
+  if (m == o.mn) a = o.sub_xn;
+  return cmul(a, pinv_prev);
+}
+__device__ __forceinline__ float2 offd_q(const Offd& o, int m, float2 pinv) {
+  float2 c = (m == 0) ? o.sup_x0 : o.sup;
+  if (m == o.mq) c = o.sup_xn;
+  return cmul(make_float2(-c.x, -c.y), pinv);
+}
+
 // Direct form: every coefficient is read where it is used (the register
 // budget of CH = 32 holds no second bank).
 
 template <int CH, bool HAS_RHS, bool SIGMA, bool NEAR, typename Ptr, typename Hook>
 __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
-                                                     int reach, const Hook& hook) {
+                                                     int reach, const Offd& od, const Hook& hook) {
   constexpr int G = kNearGroup<CH>;
   constexpr int NP = 64 * CH;
   float2 y[CH];
@@ -36,8 +80,16 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
     y[m] = r;
   }
   hook(0);
+  float2 pv[NEAR ? CH : 1];  // near: pinv of every row, for l, z and q
+  if constexpr (NEAR) {
 #pragma unroll
-  for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
+    for (int m = 0; m < CH; ++m) pv[m] = cf[A_PINV * NP + m * 64 + lane];
+#pragma unroll
+    for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, pv[m - 1]), y[m - 1], y[m]);
+  } else {
+#pragma unroll
+    for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
+  }
   hook(1);
   if constexpr (NEAR) {
     // carry = the previous chunk's local end value, on the first `reach` rows
@@ -57,10 +109,17 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
   }
+  if constexpr (NEAR) {
 #pragma unroll
-  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
+    for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], pv[m]);
 #pragma unroll
-  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
+    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, pv[m]), y[m + 1], y[m]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
+#pragma unroll
+    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
+  }
   hook(2);
   if constexpr (NEAR) {
     const float2 xin = from_next_lane(y[0]);
@@ -98,16 +157,17 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
 template <int CH, bool HAS_RHS, bool SIGMA, bool STAGED = true, bool NEAR = false, typename Ptr,
           typename Hook = NoHook>
 __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
-                                              int reach = CH, const Hook& hook = Hook()) {
+                                              int reach = CH, const Offd& od = Offd{}, const Hook& hook = Hook()) {
   if constexpr (CH > 16 || !STAGED) {
-    tri_step_lane_direct<CH, HAS_RHS, SIGMA, NEAR>(u, cf, rhs, lane, reach, hook);
+    tri_step_lane_direct<CH, HAS_RHS, SIGMA, NEAR>(u, cf, rhs, lane, reach, od, hook);
     return;
   }
   constexpr int NP = 64 * CH;
   constexpr int G = kNearGroup<CH>;
   float2 y[CH], c1[CH], c2[CH];
+  // near: c1 holds pinv for the whole step (l and q derive from it)
 #pragma unroll
-  for (int m = 0; m < CH; ++m) c1[m] = cf[A_L * NP + m * 64 + lane];
+  for (int m = 0; m < CH; ++m) c1[m] = cf[(NEAR ? A_PINV : A_L) * NP + m * 64 + lane];
 #pragma unroll
   for (int g = 0; g < CH; g += G) {
     if (!NEAR || g < reach) {
@@ -124,12 +184,19 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     y[m] = r;
   }
   hook(0);
+  if constexpr (NEAR) {
 #pragma unroll
-  for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
+    for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, c1[m - 1]), y[m - 1], y[m]);
+  } else {
+#pragma unroll
+    for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
+  }
   hook(1);
   __builtin_amdgcn_sched_barrier(0);
+  if constexpr (!NEAR) {
 #pragma unroll
-  for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
+    for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
+  }
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (NEAR) {
     const float2 yin = from_prev_lane(y[CH - 1]);
@@ -149,22 +216,34 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     for (int m = 0; m < CH; ++m) y[m] = cfma(c2[m], yin, y[m]);
   }
   __builtin_amdgcn_sched_barrier(0);
+  if constexpr (NEAR) {  // gb rows into c2; q derives from pinv in c1
 #pragma unroll
-  for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
+    for (int g = 0; g < CH; g += G) {
+      if (g + G > CH - reach) {
+#pragma unroll
+        for (int m = g; m < g + G; ++m) c2[m] = cf[A_GB * NP + m * 64 + lane];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
+  }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], c1[m]);
   __builtin_amdgcn_sched_barrier(0);
+  if constexpr (!NEAR) {
 #pragma unroll
-  for (int g = 0; g < CH; g += G) {
-    if (!NEAR || g + G > CH - reach) {
-#pragma unroll
-      for (int m = g; m < g + G; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
-    }
+    for (int m = 0; m < CH; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
   }
   __builtin_amdgcn_sched_barrier(0);
+  if constexpr (NEAR) {
 #pragma unroll
-  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
+    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, c1[m]), y[m + 1], y[m]);
+  } else {
+#pragma unroll
+    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
+  }
   hook(2);
   if constexpr (NEAR) {
     const float2 xin = from_next_lane(y[0]);
@@ -172,7 +251,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     for (int g = 0; g < CH; g += G) {
       if (g + G > CH - reach) {
 #pragma unroll
-        for (int m = g; m < g + G; ++m) y[m] = cfma(c1[m], xin, y[m]);
+        for (int m = g; m < g + G; ++m) y[m] = cfma(c2[m], xin, y[m]);
       }
     }
 #pragma unroll

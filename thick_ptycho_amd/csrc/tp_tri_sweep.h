@@ -10,9 +10,10 @@ namespace tp {
 // ------------------------------------------------------------------ persistent sweep
 
 // Whether staging vector k of a step (threads' 16-byte vectors k*NT .. k*NT+NT-1)
-// holds rows a near-carry step reads: all of l, pinv, q, sigma; of gf only the
-// first `reach` rows and of gb the last `reach` (uniform per k when a vector
-// block of NT lies inside one array, else everything is staged).
+// holds rows a near-carry step reads: all of pinv and sigma; of gf only the
+// first `reach` rows and of gb the last `reach`; none of l and q, which the step
+// derives from pinv (uniform per k when a vector block of NT lies inside one
+// array, else everything is staged).
 template <int CH, int NT>
 __device__ __forceinline__ bool stage_needed(int k, int reach) {
   constexpr int VPA = 32 * CH;  // 16-byte vectors per array
@@ -21,6 +22,7 @@ __device__ __forceinline__ bool stage_needed(int k, int reach) {
   } else {
     const int arr = (k * NT) / VPA;
     const int m0 = ((k * NT) % VPA) / 32, m1 = m0 + NT / 32;  // rows [m0, m1) of that array
+    if (arr == A_L || arr == A_Q) return false;  // derived from pinv (Offd)
     if (arr == A_GF) return m0 < reach;
     if (arr == A_GB) return m1 > CH - reach;
     return true;
@@ -63,6 +65,7 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
   const bool act = p < d.probes;
   PlaneCursor cur;
   cur.init(d, p, act);
+  const Offd od = NEAR ? make_offd(d, CH, lane) : Offd{};
   const int n = d.n;
   const int L = d.steps;
   const int row0 = lane * CH;
@@ -143,7 +146,7 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
         for (int m = 0; m < CH; ++m) rb[m] = make_float2(0.f, 0.f);
       }
     }
-    tri_step_lane<CH, RHS, SIGMA, STAGED, NEAR>(u, cf, rb, lane, reach);
+    tri_step_lane<CH, RHS, SIGMA, STAGED, NEAR>(u, cf, rb, lane, reach, od);
     const int grow = d.grad_z0 + d.grad_zdir * (s - 1);
     if (GRAD) {
       const bool skip_b = d.grad_skip_row0 && grow == 0;
@@ -224,7 +227,8 @@ __global__ __launch_bounds__(64 * PPB) void tri_sweep_kernel(const SweepArgs a) 
   if (blk < 0) return;  // padding slot of the XCD-aware order (whole workgroup)
   const int sc = find_scan(a, blk);
   const tp_tri_scan& d = a.scans[sc];
-  const int reach = d.reach ? __builtin_amdgcn_readfirstlane(*d.reach) : 64;
+  // the near-carry step needs the factors' reach and the LHS off-diagonals
+  const int reach = (d.reach && d.has_offd) ? __builtin_amdgcn_readfirstlane(*d.reach) : 64;
   tri_sweep_dispatch<CH, PPB, SIGMA, GRAD, STAGED, RHS>(a, blk, sc, reach);
 }
 

@@ -262,7 +262,7 @@ class PWEIterativeLUSolver(BasePWESolver):
                 fac = fac or facs[0]
                 self._last_factors += facs
             whole = tri_engine.TriFactors(coef.reshape(-1), fac.n, L, self.chunk, mode, fac.b_sign, fac.status,
-                                          fac.sigma, keep=[coef], waves=self.waves, wide=wide)
+                                          fac.sigma, keep=[coef], waves=self.waves, wide=wide, plan=fac.plan)
             if plain:  # the plain form shares factors within a scan: one scan per window
                 views = [tri_engine.TriFactors(coef[s], fac.n, L, self.chunk, mode, fac.b_sign, fac.status, fac.sigma,
                                                waves=self.waves, wide=wide)

@@ -95,7 +95,7 @@ class PiTPreconditioner:
         self.fac.check()
         f = self.fac
         self._one = tri_engine.TriFactors(f.coef, N, 1, self.chunk, "forward", f.b_sign, f.status, False, keep=[f],
-                                          waves=f.waves, wide=f.wide)
+                                          waves=f.waves, wide=f.wide, plan=f.plan)
         self._zero = torch.zeros((1, N), dtype=torch.complex64, device=device).expand(L, N)
 
     def block_solve(self, Xh: torch.Tensor, out: torch.Tensor) -> torch.Tensor:

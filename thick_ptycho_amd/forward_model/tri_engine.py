@@ -123,6 +123,21 @@ def factor(C: torch.Tensor, A: Tridiag, B: Tridiag, mode: str, chunk: Optional[i
                       plan=plan)
 
 
+def near_offdiagonals(plan: Optional[ModePlan], n: int):
+    """tp_tri_scan.offd for the near-carry step: the factored LHS's off-diagonals
+    (uniform sub on rows 2..n-2 and sup on rows 1..n-3, and the boundary values
+    sub[1], sub[n-1], sup[0], sup[n-2]) as 12 floats, or None when the operator
+    is not of that form."""
+    if plan is None or n < 4:
+        return None
+    sub, sup = plan.lhs.sub, plan.lhs.sup
+    s_sub, s_sup = sub[n // 2], sup[n // 2]
+    if not (np.all(sub[2:n - 1] == s_sub) and np.all(sup[1:n - 2] == s_sup)):
+        return None
+    vals = np.array([s_sub, s_sup, sub[1], sub[n - 1], sup[0], sup[n - 2]], dtype=np.complex64)
+    return [float(x) for v in vals for x in (v.real, v.imag)]
+
+
 def new_status(device) -> torch.Tensor:
     """Zeroed factor status words: [0] zero-pivot flag, [1] coupling reach
     (the rows a carry reaches in a chunk; below the chunk the sweep skips the
@@ -170,7 +185,7 @@ def factor_many(jobs, chunk: int, stream=None, out: Optional[list] = None, waves
         j.lhs_sign, j.rhs_sign = plan.lhs_sign, plan.rhs_sign
         j.n, j.ncol, j.steps, j.conj_c = n, L, L, plan.conj_c
         j.lhs_col0, j.rhs_col0, j.col_step, j.with_sigma = plan.lhs_col0, plan.rhs_col0, plan.col_step, int(plan.sigma)
-        facs.append(TriFactors(coef, n, L, chunk, mode, plan.b_sign, st, plan.sigma, keep=[base, C]))
+        facs.append(TriFactors(coef, n, L, chunk, mode, plan.b_sign, st, plan.sigma, keep=[base, C], plan=plan))
     _lib.check(_lib.load().tp_tri_factor_batch(arr, len(jobs), chunk, _lib.stream_handle(stream)),
                "tp_tri_factor_batch")
     return facs
@@ -216,6 +231,10 @@ class TriScanSpec:
         s.coef_pstride = int(self.coef_pstride)
         if self.near and not f.wide and f.status.numel() >= 2:
             s.reach = _lib.ptr(f.status) + f.status.element_size()
+            offd = near_offdiagonals(f.plan, f.n)
+            if offd is not None:
+                s.offd[:] = offd
+                s.has_offd = 1
         s.sigma = int(f.sigma)
         assert self.u0.dtype == torch.complex64 and self.u0.shape[-1] == n and self.u0.stride(-1) == 1
         s.u0 = _lib.ptr(self.u0)

@@ -263,7 +263,7 @@ def amplitude_gradient_reduced(n, probes, sqrt_i, windows, k: float, dx: float, 
                                           waves=waves if wide else None, status=status)
             first = first or facs[0]
         whole[mode] = tri_engine.TriFactors(coef.reshape(-1), W, L, chunk, mode, first.b_sign, first.status,
-                                            first.sigma, keep=[coef, facs], waves=waves, wide=wide)
+                                            first.sigma, keep=[coef, facs], waves=waves, wide=wide, plan=first.plan)
     u0 = _dev_tensor(probes, torch.complex64, device).reshape(S, W)
     U = torch.empty((S, L + 1, W), dtype=torch.complex64, device=device)
     tri_engine.sweep([tri_engine.TriScanSpec(whole["forward"], u0, out=U, coef_pstride=per)], probes_per_block=1)
