@@ -51,16 +51,68 @@ __device__ __forceinline__ Offd make_offd(const tp_tri_scan& d, int chunk, int l
   return o;
 }
 
-// l_m and q_m of this lane's row m (m a compile-time index)
-__device__ __forceinline__ float2 offd_l(const Offd& o, int m, float2 pinv_prev) {
+// sub and sup of this lane's row m (m a compile-time index), and from them
+// l_m and q_m
+__device__ __forceinline__ float2 offd_sub(const Offd& o, int m) {
   float2 a = (m == 1) ? o.sub_x1 : o.sub;
   if (m == o.mn) a = o.sub_xn;
-  return cmul(a, pinv_prev);
+  return a;
 }
-__device__ __forceinline__ float2 offd_q(const Offd& o, int m, float2 pinv) {
+__device__ __forceinline__ float2 offd_sup(const Offd& o, int m) {
   float2 c = (m == 0) ? o.sup_x0 : o.sup;
   if (m == o.mq) c = o.sup_xn;
+  return c;
+}
+__device__ __forceinline__ float2 offd_l(const Offd& o, int m, float2 pinv_prev) {
+  return cmul(offd_sub(o, m), pinv_prev);
+}
+__device__ __forceinline__ float2 offd_q(const Offd& o, int m, float2 pinv) {
+  const float2 c = offd_sup(o, m);
   return cmul(make_float2(-c.x, -c.y), pinv);
+}
+
+// Near-carry phases shared by both step forms.  Forward: elimination with
+// l_m = sub_m / piv_{m-1}, then the carry of the previous chunk's end value
+// c_{-1} through c_m = -l_m c_{m-1} (l_0 from the previous chunk's last pivot)
+// onto the first `reach` rows (rounded to groups of G).  Backward: substitution
+// with q_m = -sup_m / piv_m, then the next chunk's first value d_CH through
+// d_m = q_m d_{m+1} onto the last `reach` rows.  The carry products are taken as
+// (pinv c) sub so that the compiler cannot reuse the elimination's multipliers
+// and keep all CH of them live.
+template <int CH, int G>
+__device__ __forceinline__ void near_forward(float2 (&y)[CH], const float2 (&pv)[CH], const Offd& od, int reach) {
+#pragma unroll
+  for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, pv[m - 1]), y[m - 1], y[m]);
+  float2 c = from_prev_lane(y[CH - 1]);
+  const float2 plast = from_prev_lane(pv[CH - 1]);
+#pragma unroll
+  for (int g = 0; g < CH; g += G) {
+    if (g < reach) {
+#pragma unroll
+      for (int m = g; m < g + G; ++m) {
+        const float2 a = offd_sub(od, m);
+        c = cmul(cmul(m == 0 ? plast : pv[m - 1], c), make_float2(-a.x, -a.y));
+        y[m] = cadd(y[m], c);
+      }
+    }
+  }
+}
+template <int CH, int G>
+__device__ __forceinline__ void near_backward(float2 (&y)[CH], const float2 (&pv)[CH], const Offd& od, int reach) {
+#pragma unroll
+  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, pv[m]), y[m + 1], y[m]);
+  float2 d = from_next_lane(y[0]);
+#pragma unroll
+  for (int g = CH - G; g >= 0; g -= G) {
+    if (g + G > CH - reach) {
+#pragma unroll
+      for (int m = g + G - 1; m >= g; --m) {
+        const float2 c = offd_sup(od, m);
+        d = cmul(cmul(pv[m], d), make_float2(-c.x, -c.y));
+        y[m] = cadd(y[m], d);
+      }
+    }
+  }
 }
 
 // Direct form: every coefficient is read where it is used (the register
@@ -80,28 +132,18 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
     y[m] = r;
   }
   hook(0);
-  float2 pv[NEAR ? CH : 1];  // near: pinv of every row, for l, z and q
+  // near: pinv of every row gives l, q and the carries
+  float2 pv[NEAR ? CH : 1];
   if constexpr (NEAR) {
 #pragma unroll
     for (int m = 0; m < CH; ++m) pv[m] = cf[A_PINV * NP + m * 64 + lane];
-#pragma unroll
-    for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, pv[m - 1]), y[m - 1], y[m]);
+    near_forward<CH, G>(y, pv, od, reach);
   } else {
 #pragma unroll
     for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
   }
   hook(1);
-  if constexpr (NEAR) {
-    // carry = the previous chunk's local end value, on the first `reach` rows
-    const float2 yin = from_prev_lane(y[CH - 1]);
-#pragma unroll
-    for (int g = 0; g < CH; g += G) {
-      if (g < reach) {
-#pragma unroll
-        for (int m = g; m < g + G; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
-      }
-    }
-  } else {
+  if constexpr (!NEAR) {
     float2 A = cf[A_GF * NP + (CH - 1) * 64 + lane];
     float2 B = y[CH - 1];
     scan_affine_up(A, B);
@@ -112,8 +154,7 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
   if constexpr (NEAR) {
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], pv[m]);
-#pragma unroll
-    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, pv[m]), y[m + 1], y[m]);
+    near_backward<CH, G>(y, pv, od, reach);
   } else {
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
@@ -122,14 +163,6 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
   }
   hook(2);
   if constexpr (NEAR) {
-    const float2 xin = from_next_lane(y[0]);
-#pragma unroll
-    for (int g = 0; g < CH; g += G) {
-      if (g + G > CH - reach) {
-#pragma unroll
-        for (int m = g; m < g + G; ++m) y[m] = cfma(cf[A_GB * NP + m * 64 + lane], xin, y[m]);
-      }
-    }
 #pragma unroll
     for (int m = 0; m < CH; ++m)
       u[m] = SIGMA ? csub(y[m], u[m]) : make_float2(fmaf(2.f, y[m].x, -u[m].x), fmaf(2.f, y[m].y, -u[m].y));
@@ -165,15 +198,12 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
   constexpr int NP = 64 * CH;
   constexpr int G = kNearGroup<CH>;
   float2 y[CH], c1[CH], c2[CH];
-  // near: c1 holds pinv for the whole step (l and q derive from it)
+  // near: c1 holds pinv for the whole step (l, q and the carries derive from it)
 #pragma unroll
   for (int m = 0; m < CH; ++m) c1[m] = cf[(NEAR ? A_PINV : A_L) * NP + m * 64 + lane];
+  if constexpr (!NEAR) {
 #pragma unroll
-  for (int g = 0; g < CH; g += G) {
-    if (!NEAR || g < reach) {
-#pragma unroll
-      for (int m = g; m < g + G; ++m) c2[m] = cf[A_GF * NP + m * 64 + lane];
-    }
+    for (int m = 0; m < CH; ++m) c2[m] = cf[A_GF * NP + m * 64 + lane];
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -185,8 +215,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
   }
   hook(0);
   if constexpr (NEAR) {
-#pragma unroll
-    for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, c1[m - 1]), y[m - 1], y[m]);
+    near_forward<CH, G>(y, c1, od, reach);
   } else {
 #pragma unroll
     for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
@@ -198,16 +227,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (NEAR) {
-    const float2 yin = from_prev_lane(y[CH - 1]);
-#pragma unroll
-    for (int g = 0; g < CH; g += G) {
-      if (g < reach) {
-#pragma unroll
-        for (int m = g; m < g + G; ++m) y[m] = cfma(c2[m], yin, y[m]);
-      }
-    }
-  } else {
+  if constexpr (!NEAR) {
     float2 A = c2[CH - 1];
     float2 B = y[CH - 1];
     scan_affine_up(A, B);
@@ -216,15 +236,7 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     for (int m = 0; m < CH; ++m) y[m] = cfma(c2[m], yin, y[m]);
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (NEAR) {  // gb rows into c2; q derives from pinv in c1
-#pragma unroll
-    for (int g = 0; g < CH; g += G) {
-      if (g + G > CH - reach) {
-#pragma unroll
-        for (int m = g; m < g + G; ++m) c2[m] = cf[A_GB * NP + m * 64 + lane];
-      }
-    }
-  } else {
+  if constexpr (!NEAR) {
 #pragma unroll
     for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
   }
@@ -238,22 +250,13 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
   }
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (NEAR) {
-#pragma unroll
-    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, c1[m]), y[m + 1], y[m]);
+    near_backward<CH, G>(y, c1, od, reach);
   } else {
 #pragma unroll
     for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
   }
   hook(2);
   if constexpr (NEAR) {
-    const float2 xin = from_next_lane(y[0]);
-#pragma unroll
-    for (int g = 0; g < CH; g += G) {
-      if (g + G > CH - reach) {
-#pragma unroll
-        for (int m = g; m < g + G; ++m) y[m] = cfma(c2[m], xin, y[m]);
-      }
-    }
 #pragma unroll
     for (int m = 0; m < CH; ++m)
       u[m] = SIGMA ? csub(y[m], u[m]) : make_float2(fmaf(2.f, y[m].x, -u[m].x), fmaf(2.f, y[m].y, -u[m].y));

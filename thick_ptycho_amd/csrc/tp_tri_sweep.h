@@ -10,22 +10,18 @@ namespace tp {
 // ------------------------------------------------------------------ persistent sweep
 
 // Whether staging vector k of a step (threads' 16-byte vectors k*NT .. k*NT+NT-1)
-// holds rows a near-carry step reads: all of pinv and sigma; of gf only the
-// first `reach` rows and of gb the last `reach`; none of l and q, which the step
-// derives from pinv (uniform per k when a vector block of NT lies inside one
-// array, else everything is staged).
+// holds rows a near-carry step reads: pinv and sigma, but none of l, q, gf and
+// gb, which the step forms from pinv and the operator's off-diagonals (uniform
+// per k when a vector block of NT lies inside one array, else everything is
+// staged).
 template <int CH, int NT>
-__device__ __forceinline__ bool stage_needed(int k, int reach) {
+__device__ __forceinline__ bool stage_needed(int k) {
   constexpr int VPA = 32 * CH;  // 16-byte vectors per array
   if constexpr (NT > VPA) {
     return true;
   } else {
     const int arr = (k * NT) / VPA;
-    const int m0 = ((k * NT) % VPA) / 32, m1 = m0 + NT / 32;  // rows [m0, m1) of that array
-    if (arr == A_L || arr == A_Q) return false;  // derived from pinv (Offd)
-    if (arr == A_GF) return m0 < reach;
-    if (arr == A_GB) return m1 > CH - reach;
-    return true;
+    return arr == A_PINV || arr == A_SG;
   }
 }
 
@@ -78,7 +74,7 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
   // staging vector k of this thread, and whether the step reads it
   auto sneed = [&](int k) -> bool {
     const bool in = NV % NT == 0 || (int)threadIdx.x + k * NT < NV;
-    if constexpr (NEAR) return in && stage_needed<CH, NT>(k, reach);
+    if constexpr (NEAR) return in && stage_needed<CH, NT>(k);
     return in;
   };
 
