@@ -99,7 +99,18 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
     piv_prev = zmul(Q.a, zinv(Q.c));
   }
 
+  // l, gf, q and gb go out with non-temporal stores: a near-carry sweep reads
+  // only pinv (and sigma), which then keep their place in the caches (a sweep
+  // whose pivots were pushed out to HBM by other writes is measured 15 %
+  // slower on the config-1 launch)
   float2* o = f.coef + (size_t)s * A_COUNT * NP;
+  auto put_nt = [](float2* p, float2 v) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    f2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+  };
   double2 q[CH];
   double2 gf = one;
   bool bad = false;
@@ -116,10 +127,10 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
       q[m] = (i == n - 1) ? zero : zscale(zmul(lsup[i], pinv), -1.0);
       gf = zmul(gf, zscale(l, -1.0));
       if (zabs1(gf) > kCarryTau) reach = m + 1;
-      o[A_L * NP + pi] = to_f2(l);
-      o[A_GF * NP + pi] = to_f2(gf);
+      put_nt(o + A_L * NP + pi, to_f2(l));
+      put_nt(o + A_GF * NP + pi, to_f2(gf));
       o[A_PINV * NP + pi] = to_f2(pinv);
-      o[A_Q * NP + pi] = to_f2(q[m]);
+      put_nt(o + A_Q * NP + pi, to_f2(q[m]));
       if (f.with_sigma) {
         double2 cr = rok ? f.C[(size_t)i * f.ncol + rc] : zero;
         if (f.conj_c) cr = zconj(cr);
@@ -128,10 +139,10 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
       piv_prev = piv;
     } else {
       q[m] = zero;
-      o[A_L * NP + pi] = make_float2(0.f, 0.f);
-      o[A_GF * NP + pi] = make_float2(0.f, 0.f);
+      put_nt(o + A_L * NP + pi, make_float2(0.f, 0.f));
+      put_nt(o + A_GF * NP + pi, make_float2(0.f, 0.f));
       o[A_PINV * NP + pi] = make_float2(0.f, 0.f);
-      o[A_Q * NP + pi] = make_float2(0.f, 0.f);
+      put_nt(o + A_Q * NP + pi, make_float2(0.f, 0.f));
       if (f.with_sigma) o[A_SG * NP + pi] = make_float2(0.f, 0.f);
     }
   }
@@ -140,7 +151,7 @@ __device__ __forceinline__ void tri_factor_body(const FactorArgs& f, int blk, Mo
   for (int m = CH - 1; m >= 0; --m) {
     gb = zmul(gb, q[m]);
     if (zabs1(gb) > kCarryTau) reach = max(reach, CH - m);
-    o[A_GB * NP + m * 64 + lane] = to_f2(gb);
+    put_nt(o + A_GB * NP + m * 64 + lane, to_f2(gb));
   }
   if (f.status) {
     if (bad) f.status[0] = 1;

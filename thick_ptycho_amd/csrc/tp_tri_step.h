@@ -115,13 +115,37 @@ __device__ __forceinline__ void near_backward(float2 (&y)[CH], const float2 (&pv
   }
 }
 
+// The near-carry step once y holds the right-hand side and pv the pivots.
+template <int CH, bool SIGMA>
+__device__ __forceinline__ void near_solve(float2 (&u)[CH], float2 (&y)[CH], const float2 (&pv)[CH], const Offd& od,
+                                           int reach) {
+  constexpr int G = kNearGroup<CH>;
+  near_forward<CH, G>(y, pv, od, reach);
+#pragma unroll
+  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], pv[m]);
+  near_backward<CH, G>(y, pv, od, reach);
+#pragma unroll
+  for (int m = 0; m < CH; ++m)
+    u[m] = SIGMA ? csub(y[m], u[m]) : make_float2(fmaf(2.f, y[m].x, -u[m].x), fmaf(2.f, y[m].y, -u[m].y));
+}
+
+// Near-carry step without sigma, pivots already in registers.
+template <int CH, bool HAS_RHS>
+__device__ __forceinline__ void near_step_regs(float2 (&u)[CH], const float2 (&pv)[CH], const float2 (&rhs)[CH],
+                                               const Offd& od, int reach) {
+  float2 y[CH];
+#pragma unroll
+  for (int m = 0; m < CH; ++m)
+    y[m] = HAS_RHS ? make_float2(fmaf(0.5f, rhs[m].x, u[m].x), fmaf(0.5f, rhs[m].y, u[m].y)) : u[m];
+  near_solve<CH, false>(u, y, pv, od, reach);
+}
+
 // Direct form: every coefficient is read where it is used (the register
 // budget of CH = 32 holds no second bank).
 
 template <int CH, bool HAS_RHS, bool SIGMA, bool NEAR, typename Ptr, typename Hook>
 __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, const float2 (&rhs)[CH], int lane,
                                                      int reach, const Offd& od, const Hook& hook) {
-  constexpr int G = kNearGroup<CH>;
   constexpr int NP = 64 * CH;
   float2 y[CH];
 #pragma unroll
@@ -132,18 +156,18 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
     y[m] = r;
   }
   hook(0);
-  // near: pinv of every row gives l, q and the carries
-  float2 pv[NEAR ? CH : 1];
-  if constexpr (NEAR) {
+  if constexpr (NEAR) {  // pinv of every row gives l, q and the carries
+    float2 pv[CH];
 #pragma unroll
     for (int m = 0; m < CH; ++m) pv[m] = cf[A_PINV * NP + m * 64 + lane];
-    near_forward<CH, G>(y, pv, od, reach);
-  } else {
-#pragma unroll
-    for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
+    near_solve<CH, SIGMA>(u, y, pv, od, reach);
+    hook(3);
+    return;
   }
+#pragma unroll
+  for (int m = 1; m < CH; ++m) y[m] = cfms(cf[A_L * NP + m * 64 + lane], y[m - 1], y[m]);
   hook(1);
-  if constexpr (!NEAR) {
+  {
     float2 A = cf[A_GF * NP + (CH - 1) * 64 + lane];
     float2 B = y[CH - 1];
     scan_affine_up(A, B);
@@ -151,22 +175,12 @@ __device__ __forceinline__ void tri_step_lane_direct(float2 (&u)[CH], Ptr cf, co
 #pragma unroll
     for (int m = 0; m < CH; ++m) y[m] = cfma(cf[A_GF * NP + m * 64 + lane], yin, y[m]);
   }
-  if constexpr (NEAR) {
 #pragma unroll
-    for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], pv[m]);
-    near_backward<CH, G>(y, pv, od, reach);
-  } else {
+  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
 #pragma unroll
-    for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], cf[A_PINV * NP + m * 64 + lane]);
-#pragma unroll
-    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
-  }
+  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(cf[A_Q * NP + m * 64 + lane], y[m + 1], y[m]);
   hook(2);
-  if constexpr (NEAR) {
-#pragma unroll
-    for (int m = 0; m < CH; ++m)
-      u[m] = SIGMA ? csub(y[m], u[m]) : make_float2(fmaf(2.f, y[m].x, -u[m].x), fmaf(2.f, y[m].y, -u[m].y));
-  } else {
+  {
     float2 A = cf[A_GB * NP + lane];
     float2 B = y[0];
     scan_affine_down(A, B, lane);
@@ -196,7 +210,6 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     return;
   }
   constexpr int NP = 64 * CH;
-  constexpr int G = kNearGroup<CH>;
   float2 y[CH], c1[CH], c2[CH];
   // near: c1 holds pinv for the whole step (l, q and the carries derive from it)
 #pragma unroll
@@ -215,19 +228,18 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
   }
   hook(0);
   if constexpr (NEAR) {
-    near_forward<CH, G>(y, c1, od, reach);
-  } else {
-#pragma unroll
-    for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
+    near_solve<CH, SIGMA>(u, y, c1, od, reach);
+    hook(3);
+    return;
   }
+#pragma unroll
+  for (int m = 1; m < CH; ++m) y[m] = cfms(c1[m], y[m - 1], y[m]);
   hook(1);
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (!NEAR) {
 #pragma unroll
-    for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
-  }
+  for (int m = 0; m < CH; ++m) c1[m] = cf[A_PINV * NP + m * 64 + lane];
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (!NEAR) {
+  {
     float2 A = c2[CH - 1];
     float2 B = y[CH - 1];
     scan_affine_up(A, B);
@@ -236,31 +248,19 @@ __device__ __forceinline__ void tri_step_lane(float2 (&u)[CH], Ptr cf, const flo
     for (int m = 0; m < CH; ++m) y[m] = cfma(c2[m], yin, y[m]);
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (!NEAR) {
 #pragma unroll
-    for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
-  }
+  for (int m = 0; m < CH; ++m) c2[m] = cf[A_Q * NP + m * 64 + lane];
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], c1[m]);
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (!NEAR) {
 #pragma unroll
-    for (int m = 0; m < CH; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
-  }
+  for (int m = 0; m < CH; ++m) c1[m] = cf[A_GB * NP + m * 64 + lane];
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (NEAR) {
-    near_backward<CH, G>(y, c1, od, reach);
-  } else {
 #pragma unroll
-    for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
-  }
+  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(c2[m], y[m + 1], y[m]);
   hook(2);
-  if constexpr (NEAR) {
-#pragma unroll
-    for (int m = 0; m < CH; ++m)
-      u[m] = SIGMA ? csub(y[m], u[m]) : make_float2(fmaf(2.f, y[m].x, -u[m].x), fmaf(2.f, y[m].y, -u[m].y));
-  } else {
+  {
     float2 A = c1[0];
     float2 B = y[0];
     scan_affine_down(A, B, lane);

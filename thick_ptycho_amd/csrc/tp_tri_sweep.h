@@ -25,10 +25,34 @@ __device__ __forceinline__ bool stage_needed(int k) {
   }
 }
 
+// Blocks of 64 * CH values ahead of the plane tiles in LDS: the staged
+// coefficient arrays, or one pivot buffer per wavefront in the wave mode.
+template <int PPB, bool SIGMA, bool GRAD>
+constexpr int coef_slots() {
+  constexpr int na = SIGMA ? 6 : 5;
+  return (!SIGMA && !GRAD && PPB > na) ? PPB : na;
+}
+
 template <int CH, int PPB, bool SIGMA, bool GRAD>
 constexpr size_t sweep_body_lds() {
-  constexpr int NP = 64 * CH, NA = SIGMA ? 6 : 5, TILE = NP + NP / 8;
-  return ((size_t)NA * NP + (size_t)(PPB + (GRAD ? 1 : 0)) * TILE) * sizeof(float2);
+  constexpr int NP = 64 * CH, TILE = NP + NP / 8;
+  return ((size_t)coef_slots<PPB, SIGMA, GRAD>() * NP + (size_t)(PPB + (GRAD ? 1 : 0)) * TILE) * sizeof(float2);
+}
+
+// s_waitcnt operands (gfx9 encoding): every counter at zero; LDS only.
+constexpr int kWaitAll = 0, kWaitLds = 0xC07F;
+
+// One wavefront copies NP pivots (8 * NP bytes) from global memory into its
+// LDS buffer without passing through registers: 16 bytes per lane and
+// instruction, 1 KiB per instruction.
+template <int NP>
+__device__ __forceinline__ void pivots_to_lds(const float2* src, float2* buf, int lane) {
+  static_assert(NP % 128 == 0, "whole 1 KiB blocks");
+  const f4v* s = reinterpret_cast<const f4v*>(src) + lane;
+#pragma unroll
+  for (int i = 0; i < NP / 128; ++i)
+    __builtin_amdgcn_global_load_lds((const void*)(s + 64 * i),
+                                     (__attribute__((address_space(3))) void*)(buf + 128 * i), 16, 0, 0);
 }
 
 // One workgroup = PPB wavefronts = PPB probes of one scan walking all steps in
@@ -42,8 +66,15 @@ constexpr size_t sweep_body_lds() {
 // wavefront's tile in its share of the idle block, 3.9-4.3 ms; this form,
 // 3.5 ms.)  RHS: scans may carry an extra right-hand side (scans without one
 // take b = 0).
+//
+// Wave mode: a near-carry step without sigma or gradient reads only the
+// pivots.  Each wavefront copies the next step's into an LDS buffer of its own
+// after its plane stores (the same one-step lead and in-order wait as the
+// shared block), and the wavefronts of a workgroup step without workgroup
+// barriers.
 template <int CH, int PPB, bool SIGMA, bool GRAD, bool STAGED, bool NEAR, bool RHS>
 __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk, const int sc, const int reach) {
+  constexpr bool WAVE = NEAR && !SIGMA && !GRAD;
   constexpr int NP = 64 * CH;
   constexpr int NT = 64 * PPB;
   constexpr int NA = SIGMA ? 6 : 5;
@@ -51,6 +82,7 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
   constexpr int STRIDE_V = A_COUNT * NP / 2;  // 16-byte vectors per step in memory
   constexpr int TILE = NP + NP / 8;           // per-wavefront store tile (float2)
   constexpr int PER = (NV + NT - 1) / NT;
+  constexpr int NS = coef_slots<PPB, SIGMA, GRAD>();
   extern __shared__ f4v smem_v[];
   float2* const smem = reinterpret_cast<float2*>(smem_v);
 
@@ -69,8 +101,9 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
   const f4v* coef4 = reinterpret_cast<const f4v*>(d.coef) + (PPB == 1 ? (size_t)(act ? p : 0) * (d.coef_pstride / 2) : 0);
   // LDS: the step's coefficient block, the PPB plane tiles, the gradient row
   const float2* const cf = smem;
-  float2* const tile = smem + NA * NP + wave * TILE;
-  float2* const gbuf = smem + NA * NP + PPB * TILE;
+  float2* const tile = smem + NS * NP + wave * TILE;
+  float2* const gbuf = smem + NS * NP + PPB * TILE;
+  float2* const pbuf = smem + wave * NP;  // wave mode: this wavefront's pivots
   // staging vector k of this thread, and whether the step reads it
   auto sneed = [&](int k) -> bool {
     const bool in = NV % NT == 0 || (int)threadIdx.x + k * NT < NV;
@@ -79,12 +112,16 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
   };
 
   f4v st[PER];
+  if constexpr (WAVE) {
+    pivots_to_lds<NP>(reinterpret_cast<const float2*>(coef4) + A_PINV * NP, pbuf, lane);
+  } else {
 #pragma unroll
-  for (int k = 0; k < PER; ++k)
-    if (sneed(k)) st[k] = coef4[threadIdx.x + k * NT];
+    for (int k = 0; k < PER; ++k)
+      if (sneed(k)) st[k] = coef4[threadIdx.x + k * NT];
 #pragma unroll
-  for (int k = 0; k < PER; ++k)
-    if (sneed(k)) smem_v[threadIdx.x + k * NT] = st[k];
+    for (int k = 0; k < PER; ++k)
+      if (sneed(k)) smem_v[threadIdx.x + k * NT] = st[k];
+  }
 
   // The initial plane arrives through the tile: coalesced, and the state
   // registers never hold a pending global load when the step loop starts
@@ -114,13 +151,29 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
 
   float2 rb[CH];
   for (int s = 1; s <= L; ++s) {
+    float2 pv[WAVE ? CH : 1];
+    if constexpr (WAVE) {
+      // this step's pivots, copied a step ago behind the previous plane's
+      // stores: the wait covers stores that have had the whole step to drain
+      __builtin_amdgcn_s_waitcnt(kWaitAll);
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int m = 0; m < CH; ++m) pv[m] = pbuf[m * 64 + lane];
+    }
     {
       tile_put<CH>(tile, u, row0);
       float2* dst[3];
       cur.next(s - 1, dst);
       write_plane<CH>(tile, dst, n, lane);
     }
-    if (s < L) {
+    if constexpr (WAVE) {
+      if (s < L) {  // the pivot reads above are done before the copy lands
+        __builtin_amdgcn_s_waitcnt(kWaitLds);
+        asm volatile("" ::: "memory");
+        pivots_to_lds<NP>(reinterpret_cast<const float2*>(coef4 + (size_t)s * STRIDE_V) + A_PINV * NP, pbuf, lane);
+      }
+    }
+    if (!WAVE && s < L) {
       const f4v* src = coef4 + (size_t)s * STRIDE_V;
 #pragma unroll
       for (int k = 0; k < PER; ++k)
@@ -142,7 +195,14 @@ __device__ __forceinline__ void tri_sweep_body(const SweepArgs& a, const int blk
         for (int m = 0; m < CH; ++m) rb[m] = make_float2(0.f, 0.f);
       }
     }
-    tri_step_lane<CH, RHS, SIGMA, STAGED, NEAR>(u, cf, rb, lane, reach, od);
+    if constexpr (WAVE)
+      near_step_regs<CH, RHS>(u, pv, rb, od, reach);
+    else
+      tri_step_lane<CH, RHS, SIGMA, STAGED, NEAR>(u, cf, rb, lane, reach, od);
+    if constexpr (WAVE) {  // the tile is this wavefront's own: the next step's writes follow its reads
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
     const int grow = d.grad_z0 + d.grad_zdir * (s - 1);
     if (GRAD) {
       const bool skip_b = d.grad_skip_row0 && grow == 0;
