@@ -205,11 +205,12 @@ typedef struct tp_tri_scan {
    * synchronisation.  tp_tri_step and tp_tri_sweep_wide ignore it. */
   const int* reach;
   /* The LHS off-diagonals of the factored operator, which the near-carry step
-   * needs (it derives the multipliers l_i = sub_i / piv_{i-1} and
-   * q_i = -sup_i / piv_i from the stored 1/piv instead of staging them): with
-   * has_offd != 0, sub is uniform on rows 2 .. n-2 and sup on rows 1 .. n-3,
-   * and offd holds complex64 {sub interior, sup interior, sub[1], sub[n-1],
-   * sup[0], sup[n-2]} as 12 floats (every mode's operator has this form, n >= 4).
+   * needs (it eliminates with the stored 1/piv and these instead of staging the
+   * multipliers): with has_offd != 0, sub is uniform on rows 2 .. n-2 and sup
+   * on rows 1 .. n-3, every one of them is purely imaginary, and offd holds
+   * complex64 {sub interior, sup interior, sub[1], sub[n-1], sup[0], sup[n-2]}
+   * as 12 floats whose real parts are zero (the step reads the imaginary parts;
+   * every mode's CN operator with real k, dx, dz has this form, n >= 4).
    * has_offd = 0: no near-carry step. */
   float offd[12];
   int has_offd;

@@ -6,7 +6,9 @@ per workgroup cut coefficient traffic and barriers per probe, fewer give more
 workgroups to spread over the 256 CUs and more wavefronts per SIMD.  The best
 value depends on the widths, the probe counts and the number of batches in the
 launch, so it is measured: each candidate runs the real launch (outputs are
-simply rewritten), the median of a few repetitions decides, and the choice is
+simply rewritten), in interleaved rounds after one warm-up round of all of them
+(a GPU that is still raising its clocks would otherwise favour the candidates
+timed last), the median per candidate decides, and the choice is
 cached per workload shape and device for the process (and in a JSON file when
 ``THICK_PTYCHO_AMD_TUNE_FILE`` names one).
 """
@@ -52,7 +54,7 @@ def candidates(specs) -> Tuple[int, ...]:
     return (1, 2) if chunk >= 32 else (1, 2, 4, 8)
 
 
-def tune_sweep(specs: Sequence["tri_engine.TriScanSpec"], reps: int = 3) -> dict:
+def tune_sweep(specs: Sequence["tri_engine.TriScanSpec"], reps: int = 5) -> dict:
     """{'ppb': best probes per workgroup, 'ms': {ppb: median ms}} for this launch shape."""
     if not _CACHE:
         _load_file()
@@ -60,17 +62,18 @@ def tune_sweep(specs: Sequence["tri_engine.TriScanSpec"], reps: int = 3) -> dict
     if key in _CACHE:
         return _CACHE[key]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    times = {}
-    for ppb in candidates(specs):
-        tri_engine.sweep(specs, probes_per_block=ppb)  # warm-up
-        ts = []
-        for _ in range(reps):
+    cands = candidates(specs)
+    for ppb in cands:  # warm-up round
+        tri_engine.sweep(specs, probes_per_block=ppb)
+    ts = {ppb: [] for ppb in cands}
+    for _ in range(reps):
+        for ppb in cands:
             e0.record()
             tri_engine.sweep(specs, probes_per_block=ppb)
             e1.record()
             torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1))
-        times[ppb] = sorted(ts)[len(ts) // 2]
+            ts[ppb].append(e0.elapsed_time(e1))
+    times = {ppb: sorted(v)[len(v) // 2] for ppb, v in ts.items()}
     best = min(times, key=times.get)
     res = {"ppb": best, "ms": {str(k): v for k, v in times.items()}}
     _CACHE[key] = res

@@ -19,28 +19,27 @@ template <int CH>
 constexpr int kNearGroup = CH < 4 ? CH : 4;
 constexpr int kNearMinChunk = 8;
 
-// The near-carry step derives the multipliers from the pivots instead of
-// staging them: l_i = sub_i pinv_{i-1} and q_i = -sup_i pinv_i, with the LHS
-// off-diagonals uniform on interior rows (tp_tri_scan.offd).  Each array has one
-// boundary value at either end (rows 1 and n-1 of sub, 0 and n-2 of sup); a
+// The near-carry step works from the pivots alone: with the LHS off-diagonals
+// i * sub_i and i * sup_i (purely imaginary, uniform on interior rows,
+// tp_tri_scan.offd), forward elimination is z_m = pinv_m (b_m - i sub_m z_{m-1})
+// and back substitution x_m = z_m - pinv_m (i sup_m x_{m+1}).  Each array has
+// one boundary value at either end (rows 1 and n-1 of sub, 0 and n-2 of sup); a
 // lane keeps the values of its own rows: the first lane's, and at the uniform
 // rows mn = (n-1) % CH, mq = (n-2) % CH those of the lanes holding rows n-1 and
-// n-2.  A boundary row that starts or ends a chunk takes its multiplier through
-// the stored carry factors (gf, gb) instead.
+// n-2.
 struct Offd {
-  float2 sub, sup;        // interior rows
-  float2 sub_x1, sup_x0;  // this lane at m = 1 (sub) / m = 0 (sup)
-  float2 sub_xn, sup_xn;  // this lane at m = mn (sub) / m = mq (sup)
+  float sub, sup;        // interior rows (imaginary parts)
+  float sub_x1, sup_x0;  // this lane at m = 1 (sub) / m = 0 (sup)
+  float sub_xn, sup_xn;  // this lane at m = mn (sub) / m = mq (sup)
   int mn, mq;
 };
 
 __device__ __forceinline__ Offd make_offd(const tp_tri_scan& d, int chunk, int lane) {
-  const float* v = d.offd;
+  const float* v = d.offd;  // (re, im) pairs; the real parts are zero
   Offd o;
-  o.sub = make_float2(v[0], v[1]);
-  o.sup = make_float2(v[2], v[3]);
-  const float2 sub1 = make_float2(v[4], v[5]), subn = make_float2(v[6], v[7]);
-  const float2 sup0 = make_float2(v[8], v[9]), supn = make_float2(v[10], v[11]);
+  o.sub = v[1];
+  o.sup = v[3];
+  const float sub1 = v[5], subn = v[7], sup0 = v[9], supn = v[11];
   const int n = d.n;
   o.mn = (n - 1) % chunk;
   o.mq = n >= 2 ? (n - 2) % chunk : -1;
@@ -51,47 +50,41 @@ __device__ __forceinline__ Offd make_offd(const tp_tri_scan& d, int chunk, int l
   return o;
 }
 
-// sub and sup of this lane's row m (m a compile-time index), and from them
-// l_m and q_m
-__device__ __forceinline__ float2 offd_sub(const Offd& o, int m) {
-  float2 a = (m == 1) ? o.sub_x1 : o.sub;
+// imaginary parts of sub and sup at this lane's row m (m a compile-time index)
+__device__ __forceinline__ float offd_sub(const Offd& o, int m) {
+  float a = (m == 1) ? o.sub_x1 : o.sub;
   if (m == o.mn) a = o.sub_xn;
   return a;
 }
-__device__ __forceinline__ float2 offd_sup(const Offd& o, int m) {
-  float2 c = (m == 0) ? o.sup_x0 : o.sup;
+__device__ __forceinline__ float offd_sup(const Offd& o, int m) {
+  float c = (m == 0) ? o.sup_x0 : o.sup;
   if (m == o.mq) c = o.sup_xn;
   return c;
 }
-__device__ __forceinline__ float2 offd_l(const Offd& o, int m, float2 pinv_prev) {
-  return cmul(offd_sub(o, m), pinv_prev);
+// c - i b z and i b z (b real)
+__device__ __forceinline__ float2 cfms_i(float b, float2 z, float2 c) {
+  return make_float2(fmaf(b, z.y, c.x), fmaf(-b, z.x, c.y));
 }
-__device__ __forceinline__ float2 offd_q(const Offd& o, int m, float2 pinv) {
-  const float2 c = offd_sup(o, m);
-  return cmul(make_float2(-c.x, -c.y), pinv);
-}
+__device__ __forceinline__ float2 cmul_i(float b, float2 z) { return make_float2(-b * z.y, b * z.x); }
 
-// Near-carry phases shared by both step forms.  Forward: elimination with
-// l_m = sub_m / piv_{m-1}, then the carry of the previous chunk's end value
-// c_{-1} through c_m = -l_m c_{m-1} (l_0 from the previous chunk's last pivot)
-// onto the first `reach` rows (rounded to groups of G).  Backward: substitution
-// with q_m = -sup_m / piv_m, then the next chunk's first value d_CH through
-// d_m = q_m d_{m+1} onto the last `reach` rows.  The carry products are taken as
-// (pinv c) sub so that the compiler cannot reuse the elimination's multipliers
-// and keep all CH of them live.
+// Near-carry phases shared by both step forms.  Forward: elimination within
+// the chunk, then the carry of the previous chunk's last z: its increment
+// dz_{-1} = z_in goes through dz_m = pinv_m (-i sub_m dz_{m-1}) onto the first
+// `reach` rows (rounded to groups of G).  Backward: substitution within the
+// chunk, then the next chunk's first x through dx_m = pinv_m (-i sup_m
+// dx_{m+1}) onto the last `reach` rows.
 template <int CH, int G>
 __device__ __forceinline__ void near_forward(float2 (&y)[CH], const float2 (&pv)[CH], const Offd& od, int reach) {
+  y[0] = cmul(pv[0], y[0]);
 #pragma unroll
-  for (int m = 1; m < CH; ++m) y[m] = cfms(offd_l(od, m, pv[m - 1]), y[m - 1], y[m]);
+  for (int m = 1; m < CH; ++m) y[m] = cmul(pv[m], cfms_i(offd_sub(od, m), y[m - 1], y[m]));
   float2 c = from_prev_lane(y[CH - 1]);
-  const float2 plast = from_prev_lane(pv[CH - 1]);
 #pragma unroll
   for (int g = 0; g < CH; g += G) {
     if (g < reach) {
 #pragma unroll
       for (int m = g; m < g + G; ++m) {
-        const float2 a = offd_sub(od, m);
-        c = cmul(cmul(m == 0 ? plast : pv[m - 1], c), make_float2(-a.x, -a.y));
+        c = cmul(pv[m], cmul_i(-offd_sub(od, m), c));
         y[m] = cadd(y[m], c);
       }
     }
@@ -100,15 +93,14 @@ __device__ __forceinline__ void near_forward(float2 (&y)[CH], const float2 (&pv)
 template <int CH, int G>
 __device__ __forceinline__ void near_backward(float2 (&y)[CH], const float2 (&pv)[CH], const Offd& od, int reach) {
 #pragma unroll
-  for (int m = CH - 2; m >= 0; --m) y[m] = cfma(offd_q(od, m, pv[m]), y[m + 1], y[m]);
+  for (int m = CH - 2; m >= 0; --m) y[m] = cfms(pv[m], cmul_i(offd_sup(od, m), y[m + 1]), y[m]);
   float2 d = from_next_lane(y[0]);
 #pragma unroll
   for (int g = CH - G; g >= 0; g -= G) {
     if (g + G > CH - reach) {
 #pragma unroll
       for (int m = g + G - 1; m >= g; --m) {
-        const float2 c = offd_sup(od, m);
-        d = cmul(cmul(pv[m], d), make_float2(-c.x, -c.y));
+        d = cmul(pv[m], cmul_i(-offd_sup(od, m), d));
         y[m] = cadd(y[m], d);
       }
     }
@@ -121,8 +113,6 @@ __device__ __forceinline__ void near_solve(float2 (&u)[CH], float2 (&y)[CH], con
                                            int reach) {
   constexpr int G = kNearGroup<CH>;
   near_forward<CH, G>(y, pv, od, reach);
-#pragma unroll
-  for (int m = 0; m < CH; ++m) y[m] = cmul(y[m], pv[m]);
   near_backward<CH, G>(y, pv, od, reach);
 #pragma unroll
   for (int m = 0; m < CH; ++m)

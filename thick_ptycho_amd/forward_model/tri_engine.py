@@ -127,12 +127,17 @@ def near_offdiagonals(plan: Optional[ModePlan], n: int):
     """tp_tri_scan.offd for the near-carry step: the factored LHS's off-diagonals
     (uniform sub on rows 2..n-2 and sup on rows 1..n-3, and the boundary values
     sub[1], sub[n-1], sup[0], sup[n-2]) as 12 floats, or None when the operator
-    is not of that form."""
+    is not of that form.  The step multiplies by them as purely imaginary
+    numbers, which the CN operators of every boundary condition and mode are
+    (the paraxial term i dz / (4 k dx^2) with real k, dx, dz); any real part
+    leaves the scan on the full carries."""
     if plan is None or n < 4:
         return None
     sub, sup = plan.lhs.sub, plan.lhs.sup
     s_sub, s_sup = sub[n // 2], sup[n // 2]
     if not (np.all(sub[2:n - 1] == s_sub) and np.all(sup[1:n - 2] == s_sup)):
+        return None
+    if np.any(sub[1:n].real != 0) or np.any(sup[:n - 1].real != 0):
         return None
     vals = np.array([s_sub, s_sup, sub[1], sub[n - 1], sup[0], sup[n - 2]], dtype=np.complex64)
     return [float(x) for v in vals for x in (v.real, v.imag)]
