@@ -33,10 +33,10 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the persistent sweep on this workload (nz = 1024,
-# 4 probes per workgroup, the 384-wide scan at 8 rows per lane), from rocprofv3
-# FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy in the same process
+# 4 or 8 probes per workgroup, the 384-wide scan at 8 rows per lane), from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy in the same process
 # (profiles/tri_sweep_batched_cfg1_traffic.md).
-MEASURED_SWEEP_TRAFFIC = {(1024, 4, True): 12.78e9}
+MEASURED_SWEEP_TRAFFIC = {(1024, 4, True): 12.78e9, (1024, 8, True): 12.74e9}
 
 
 def log(msg: str) -> None:
