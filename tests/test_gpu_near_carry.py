@@ -153,3 +153,46 @@ def test_boundary_rows_of_the_derived_multipliers(gpu, rng, bc, mode, nx):
         if mode == "adjoint":
             got = got[:, :, ::-1]
         assert rel_l2(got, ref) < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["forward", "adjoint"])
+def test_wave_mode_at_every_batch_shape(gpu, rng, mode):
+    """The near step without sigma or gradient runs in the wave mode: each
+    wavefront copies its pivots into an LDS buffer of its own and the
+    wavefronts of a workgroup step without barriers.  At 1, 2, 4 and 8 probes
+    per workgroup (8 buffers outnumber the shared coefficient block's slots),
+    with a scan at half the launch's chunk and probe counts that leave the last
+    workgroup part-filled: full fields bitwise equal across the batch shapes,
+    within 1e-6 of the full carries and 1e-4 of the FP64 oracle."""
+    print(f"case wave mode {mode}", flush=True)
+    k, bc = 2 * np.pi, "impedance"
+    cases = []
+    for nx, P in ((1000, 13), (384, 6)):  # 16 and 8 rows per lane
+        g = pwe.Grid(dimension=2, nx=nx, nz=40, dx=2.0, dz=4.0, k=k)
+        n = random_object(nx, 40, rng, amp=1e-3)
+        probes = rng.standard_normal((P, nx)) + 1j * rng.standard_normal((P, nx))
+        A, B = operators.step_operators_1d(nx, k, 2.0, 4.0, bc)
+        C = torch.from_numpy(pwe.object_contribution(g, n)).to("cuda")
+        fac = tri_engine.factor(C, A, B, mode, chunk=operators.chunk_for(nx))
+        assert int(fac.status[1].item()) < fac.chunk
+        u0 = torch.from_numpy(probes).to("cuda", torch.complex64)
+        cases.append((g, n, probes, fac, u0))
+    assert [c[3].chunk for c in cases] == [16, 8]
+
+    def run(ppb, near=True):
+        outs = [torch.empty((c[4].shape[0], c[0].nz, c[0].nx), dtype=torch.complex64, device="cuda") for c in cases]
+        tri_engine.sweep([tri_engine.TriScanSpec(c[3], c[4], out=o, near=near) for c, o in zip(cases, outs)],
+                         probes_per_block=ppb)
+        return [o.cpu().numpy() for o in outs]
+
+    base = run(4)
+    for ppb in (1, 2, 8):
+        for got, ref in zip(run(ppb), base):
+            np.testing.assert_array_equal(got, ref)
+    for got, full, (g, n, probes, _, _) in zip(base, run(4, near=False), cases):
+        assert rel_l2(got, full) < 1e-6
+        ref = pwe.solve(g, bc, n, probes, mode)
+        got = got.transpose(0, 2, 1)
+        if mode == "adjoint":
+            got = got[:, :, ::-1]
+        assert rel_l2(got, ref) < 1e-4
