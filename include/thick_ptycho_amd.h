@@ -202,7 +202,8 @@ typedef struct tp_tri_scan {
    * to the first / last *reach rows only, with no 64-lane scan; the terms it
    * leaves out are below 2^-32 of the carried values.  Read on the device when
    * the sweep starts, so a sweep queued behind its factorisation needs no host
-   * synchronisation.  tp_tri_step and tp_tri_sweep_wide ignore it. */
+   * synchronisation.  tp_tri_step takes the same near-carry step (reading the
+   * word at every launch); tp_tri_sweep_wide ignores it. */
   const int* reach;
   /* The LHS off-diagonals of the factored operator, which the near-carry step
    * needs (it eliminates with the stored 1/piv and these instead of staging the

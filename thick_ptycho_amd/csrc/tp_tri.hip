@@ -227,13 +227,33 @@ __global__ __launch_bounds__(256) void tri_step_kernel(const SweepArgs a, int s)
   }
   float2 rb[CH];
   const float2* cf = (const float2*)d.coef + (size_t)(s - 1) * A_COUNT * NP;
-  if (d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps)) {
+  const bool has_rhs = d.rhs != nullptr && (d.rhs_steps == 0 || s <= d.rhs_steps);
+  if (has_rhs) {
     const float2* r = (const float2*)d.rhs + (size_t)p * d.rhs_pstride + (int64_t)(s - 1) * d.rhs_sstride;
 #pragma unroll
     for (int m = 0; m < CH; ++m) rb[m] = (row0 + m < n) ? r[row0 + m] : make_float2(0.f, 0.f);
-    tri_step_lane<CH, true, SIGMA>(u, cf, rb, lane);
-  } else {
-    tri_step_lane<CH, false, SIGMA>(u, cf, rb, lane);
+  }
+  // near-carry step as in the persistent sweep (the factors' reach and the LHS
+  // off-diagonals): only the pivots are read, no 64-lane scans
+  bool near = false;
+  if constexpr (CH >= kNearMinChunk) {
+    const int reach = (d.reach && d.has_offd) ? __builtin_amdgcn_readfirstlane(*d.reach) : CH;
+    if (reach < CH) {
+      near = true;
+      constexpr int G = kNearGroup<CH>;
+      const int rr = (reach + G - 1) / G * G;
+      const Offd od = make_offd(d, CH, lane);
+      if (has_rhs)
+        tri_step_lane<CH, true, SIGMA, true, true>(u, cf, rb, lane, rr, od);
+      else
+        tri_step_lane<CH, false, SIGMA, true, true>(u, cf, rb, lane, rr, od);
+    }
+  }
+  if (!near) {
+    if (has_rhs)
+      tri_step_lane<CH, true, SIGMA>(u, cf, rb, lane);
+    else
+      tri_step_lane<CH, false, SIGMA>(u, cf, rb, lane);
   }
   tile_put<CH>(tile, u, row0);
   PlaneWriter<CH> w;
