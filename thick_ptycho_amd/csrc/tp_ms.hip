@@ -28,12 +28,23 @@ __device__ __forceinline__ int ms_find_scan(const MsArgs& a, int blk) {
   return sc;
 }
 
+// Field planes stream out with non-temporal stores: they are not read back by
+// the sweep, and ordinary stores push the transmission and propagator out of
+// the caches (split form at 512 x 1024 x 1023: 1.80e8 -> 2.51e8 steps/s).
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+
 template <int R>
 __device__ __forceinline__ void ms_store(const tp_ms_scan& d, int p, int z, const float2* x, int lane) {
   if (d.out) {
     float2* o = (float2*)d.out + (size_t)p * d.out_pstride + (int64_t)z * d.out_zstride;
 #pragma unroll
-    for (int m = 0; m < R; ++m) o[lane + 64 * m] = x[m];
+    for (int m = 0; m < R; ++m) store_nt(o + lane + 64 * m, x[m]);
   }
   if (d.exitp && z == d.steps) {
     float2* o = (float2*)d.exitp + (size_t)p * d.exit_pstride;
@@ -220,7 +231,7 @@ __global__ __launch_bounds__(64 * S) void ms_split_kernel(const MsArgs a) {
 #pragma unroll
       for (int t = 0; t < QL; ++t)
 #pragma unroll
-        for (int r = 0; r < S; ++r) o[pos(t, r)] = xs[t][r];
+        for (int r = 0; r < S; ++r) store_nt(o + pos(t, r), xs[t][r]);
     }
 #pragma unroll
     for (int t = 0; t < QL; ++t)
