@@ -171,8 +171,8 @@ __device__ __forceinline__ void wide_store(float2* o, const float2* tile, int wr
   for (int k = 0; k < CH / 2; ++k) {
     const int j = 2 * (lane + 64 * k);
     const int r = wrow + j;
-    if (aligned && r + 1 < n) {
-      *reinterpret_cast<f4v*>(o + r) = *reinterpret_cast<const f4v*>(tile + tile_index(j));
+    if (aligned && r + 1 < n) {  // planes stream out: non-temporal stores
+      __builtin_nontemporal_store(*reinterpret_cast<const f4v*>(tile + tile_index(j)), reinterpret_cast<f4v*>(o + r));
     } else {
       if (r < n) o[r] = tile[tile_index(j)];
       if (r + 1 < n) o[r + 1] = tile[tile_index(j + 1)];
