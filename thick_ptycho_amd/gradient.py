@@ -24,7 +24,7 @@ backward runs the same checkpointed exact adjoint.
 from __future__ import annotations
 
 import math
-from typing import Tuple
+from typing import Optional, Tuple
 
 import numpy as np
 import torch
@@ -98,12 +98,21 @@ REFINE_STEPS = 2
 class _Sweeper:
     """Forward factors + adjoint factors of one object, and the checkpointed sweeps."""
 
-    def __init__(self, C: torch.Tensor, A, B, adjoint_mode: str, checkpoint: int, ppb: int = 4):
+    def __init__(self, C: torch.Tensor, A, B, adjoint_mode: str, checkpoint: int, ppb: Optional[int] = None):
         self.n, self.L = C.shape
         self.fwd = tri_engine.factor(C, A, B, "forward")
         self.adj = tri_engine.factor(C, A, B, adjoint_mode)
         self.K = max(1, min(checkpoint if checkpoint > 0 else self.L, self.L))
-        self.ppb = ppb
+        self._ppb = ppb
+        self._cus = torch.cuda.get_device_properties(C.device).multi_processor_count
+
+    def ppb_for(self, P: int) -> int:
+        """Probes per workgroup for a sweep of P probes (``ppb`` when given): 4
+        when workgroups of 4 still cover every compute unit, else 1 (a few
+        hundred probes in workgroups of 4 would leave most units idle)."""
+        if self._ppb is not None:
+            return self._ppb
+        return 4 if (P + 3) // 4 >= self._cus else 1
 
     def refine_exit(self, ck: torch.Tensor, e0: torch.Tensor, steps: int = REFINE_STEPS) -> list:
         """Corrections [e1_L, ..., e_L] of the exit plane by ``steps`` refinement
@@ -123,13 +132,13 @@ class _Sweeper:
             ln = min(s0 + K, L) - s0
             fw = tri_engine.step_slice(self.fwd, s0, ln)
             views = [q[:, : ln + 1] for q in segs]
-            tri_engine.sweep([tri_engine.TriScanSpec(fw, ck[:, g], out=views[0])], probes_per_block=self.ppb)
+            tri_engine.sweep([tri_engine.TriScanSpec(fw, ck[:, g], out=views[0])], probes_per_block=self.ppb_for(P))
             for k in range(1, steps + 1):
                 r = tri_engine.residual(fw, views[:k])
                 nxt = torch.empty_like(e0)
                 out = views[k] if k < steps else None
                 tri_engine.sweep([tri_engine.TriScanSpec(fw, state[k - 1], rhs=r, out=out, exit=nxt)],
-                                 probes_per_block=self.ppb)
+                                 probes_per_block=self.ppb_for(P))
                 state[k - 1] = nxt
         return state
 
@@ -139,7 +148,7 @@ class _Sweeper:
         ck = torch.empty((P, self.L // self.K + 1, self.n), dtype=torch.complex64, device=dev)
         ex = torch.empty((P, self.n), dtype=torch.complex64, device=dev)
         tri_engine.sweep([tri_engine.TriScanSpec(self.fwd, u0, ckpt=ck, ckpt_every=self.K, exit=ex)],
-                         probes_per_block=self.ppb)
+                         probes_per_block=self.ppb_for(P))
         return ck, ex
 
     def adjoint(self, ck: torch.Tensor, seed: torch.Tensor, skip_row0: bool = False) -> torch.Tensor:
@@ -158,7 +167,7 @@ class _Sweeper:
             ln = s1 - s0
             segv = seg[:, : ln + 1]
             fw = tri_engine.step_slice(self.fwd, s0, ln)
-            tri_engine.sweep([tri_engine.TriScanSpec(fw, ck[:, g], out=segv)], probes_per_block=self.ppb)
+            tri_engine.sweep([tri_engine.TriScanSpec(fw, ck[:, g], out=segv)], probes_per_block=self.ppb_for(P))
             aw = tri_engine.step_slice(self.adj, L - s1, ln)
             first = s1 == L
             spec = tri_engine.TriScanSpec(
@@ -167,7 +176,8 @@ class _Sweeper:
                 gfwd=segv, gfwd_z0=ln, gfwd_zdir=-1,
                 grad=grad, grad_z0=s1 - 1, grad_zdir=-1, grad_skip_row0=skip_row0,
             )
-            tri_engine.sweep([spec], probes_per_block=self.ppb if self.ppb in (1, 4) else 4)
+            ppb = self.ppb_for(P)
+            tri_engine.sweep([spec], probes_per_block=ppb if ppb in (1, 4) else 4)
         return grad
 
 
