@@ -80,7 +80,7 @@ class PWEIterativeLUSolver(BasePWESolver):
     solver_cache_class = PWEIterativeLUSolverCache
 
     def __init__(self, simulation_space, ptycho_probes, bc_type=BoundaryType.IMPEDANCE, results_dir="",
-                 use_logging=False, verbose=False, log=None, test_bcs=None, device=None, probes_per_block=4):
+                 use_logging=False, verbose=False, log=None, test_bcs=None, device=None, probes_per_block=None):
         super().__init__(simulation_space, ptycho_probes, bc_type=bc_type, results_dir=results_dir,
                          use_logging=use_logging, verbose=verbose, log=log, test_bcs=test_bcs, device=device)
         self.pwe_finite_differences = PWEForwardModel(simulation_space, bc_type=bc_type)
@@ -164,7 +164,7 @@ class PWEIterativeLUSolver(BasePWESolver):
             return band_engine.BandScanSpec(fac, u0, **kw)
         return tri_engine.TriScanSpec(fac, u0, **kw)
 
-    def _launch(self, specs, probes_per_block: int = 4, plain: bool = False) -> None:
+    def _launch(self, specs, probes_per_block=None, plain: bool = False) -> None:
         if self.dimension == 3:
             # two probes per workgroup share each band-coefficient load once the
             # launch has enough probes to keep most CUs busy with pairs

@@ -346,10 +346,20 @@ def _structs(specs: Sequence[TriScanSpec], allow_half_chunk: bool = False):
     return (_lib.TriScan * len(specs))(*structs), chunk
 
 
-def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) -> None:
+def batch_for(probes: int, device=None) -> int:
+    """Probes per workgroup for a launch of ``probes`` probes: 4 when workgroups
+    of 4 still give every compute unit one, else 1 (a few hundred probes in
+    workgroups of 4 would leave most units idle)."""
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    return 4 if (probes + 3) // 4 >= cus else 1
+
+
+def sweep(specs: Sequence[TriScanSpec], probes_per_block: Optional[int] = None, stream=None) -> None:
     """Persistent batched sweep over all steps of all scans (one launch).  Scans
     factored with half the widest scan's chunk (16 or 32) step with that many
-    rows per lane inside the same launch.  Wide-layout scans go to
+    rows per lane inside the same launch.  ``probes_per_block`` None: from the
+    launch's probe count (``batch_for``).  Wide-layout scans go to
     tp_tri_sweep_wide, one launch per (chunk, waves)."""
     wide = [s for s in specs if s.fac.wide]
     if wide:
@@ -364,6 +374,8 @@ def sweep(specs: Sequence[TriScanSpec], probes_per_block: int = 4, stream=None) 
         specs = [s for s in specs if not s.fac.wide]
         if not specs:
             return
+    if probes_per_block is None:
+        probes_per_block = batch_for(sum(int(s.u0.shape[0]) for s in specs), specs[0].u0.device)
     arr, chunk = _structs(specs, allow_half_chunk=True)
     if chunk >= 32:
         # LDS of 2048-row blocks: two probes per workgroup, one with gradient accumulation

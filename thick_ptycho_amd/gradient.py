@@ -104,15 +104,12 @@ class _Sweeper:
         self.adj = tri_engine.factor(C, A, B, adjoint_mode)
         self.K = max(1, min(checkpoint if checkpoint > 0 else self.L, self.L))
         self._ppb = ppb
-        self._cus = torch.cuda.get_device_properties(C.device).multi_processor_count
+        self._dev = C.device
 
     def ppb_for(self, P: int) -> int:
-        """Probes per workgroup for a sweep of P probes (``ppb`` when given): 4
-        when workgroups of 4 still cover every compute unit, else 1 (a few
-        hundred probes in workgroups of 4 would leave most units idle)."""
-        if self._ppb is not None:
-            return self._ppb
-        return 4 if (P + 3) // 4 >= self._cus else 1
+        """Probes per workgroup for a sweep of P probes: ``ppb`` when given, else
+        tri_engine.batch_for."""
+        return self._ppb if self._ppb is not None else tri_engine.batch_for(P, self._dev)
 
     def refine_exit(self, ck: torch.Tensor, e0: torch.Tensor, steps: int = REFINE_STEPS) -> list:
         """Corrections [e1_L, ..., e_L] of the exit plane by ``steps`` refinement

@@ -233,7 +233,7 @@ class ReferenceNLCG:
     conjugate; alpha = -<d, grad> / sum_p |delta_p|^2."""
 
     def __init__(self, data, probes, k: float, dx: float, dz: float, bc: str = "dirichlet",
-                 phase_retrieval: bool = True, device=None, probes_per_block: int = 4):
+                 phase_retrieval: bool = True, device=None, probes_per_block=None):
         from .forward_model import operators
 
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
@@ -269,7 +269,8 @@ class ReferenceNLCG:
         spec = tri_engine.TriScanSpec(adj, lam, exit=lam, rhs=err.reshape(P, 1, N).contiguous(), rhs_steps=1,
                                       gfwd=u, gfwd_z0=L, gfwd_zdir=-1, grad=G, grad_z0=L - 1, grad_zdir=-1,
                                       grad_skip_row0=True)
-        tri_engine.sweep([spec], probes_per_block=self.ppb if self.ppb in (1, 4) else 4)
+        ppb = self.ppb if self.ppb is not None else tri_engine.batch_for(int(self.probes.shape[0]), self.dev)
+        tri_engine.sweep([spec], probes_per_block=ppb if ppb in (1, 4) else 4)
         return tg.reference_gradient_parts(G, self.k, self.dz)
 
     def _alphak(self, fwd, u: torch.Tensor, scale: complex, grad_E: torch.Tensor, d: torch.Tensor) -> float:
